@@ -1,0 +1,355 @@
+// backproject.hip -- voxel-driven cone-beam backprojector with aprict's
+// semantics (ap_backproject): the exact intersection-length branch and the
+// fast interpolating (FDK / SIRT) branch of CB_backproject.cu:313-733,
+// including sub-voxel sampling, cylinder mask, detector-edge softening,
+// Z-power weights and the summand multiplier (:952-1027).
+//
+// One thread per voxel; a wavefront covers 64 consecutive voxels along the
+// volume's z axis.  Views are processed in batches of kViewBatch: the block
+// derives each view's projective map once (FP64, rounded to FP32) into LDS.
+#include "ap_common.h"
+
+namespace {
+
+constexpr int kBz = 64, kBy = 4;
+constexpr int kViewBatch = 64;
+
+// Per-view constants (FP32) of the central projection x -> detector:
+//   cu = (Au.x + au)/(W.x + w0),  cv = (Av.x + av)/(W.x + w0)  (pixels from
+//   the detector centre, geometry.py:75-107 expanded into a projective map),
+//   lambda = K/(W.x + w0) is the source->detector / source->point ratio,
+//   Z = Zc.x + z0 is the distance along the source->detector-centre axis.
+struct ViewMap {
+    float Au[3], au, Av[3], av, W[3], w0, K;
+    float Zc[3], z0, Lnorm;
+    float S[3], D[3], U[3], Vv[3];
+    float nu, nv, uvnorm;
+};
+
+__device__ void make_view_map(const ap_proj_spec* specs, uint32_t p, ViewMap& m) {
+    const ViewGeom g = load_view(specs, p);
+    const d3 n = cross(g.U, g.V);
+    const double K = dot(g.D - g.S, n);
+    const double uu = dot(g.U, g.U), vv = dot(g.V, g.V);
+    const double a3u = dot(g.U, g.S - g.D), a3v = dot(g.V, g.S - g.D);
+    const double nS = dot(n, g.S);
+    const d3 Au = (n * a3u + g.U * K) * (1.0 / uu);
+    const d3 Av = (n * a3v + g.V * K) * (1.0 / vv);
+    m.Au[0] = (float)Au.x; m.Au[1] = (float)Au.y; m.Au[2] = (float)Au.z;
+    m.au = (float)((-a3u * nS - K * dot(g.U, g.S)) / uu);
+    m.Av[0] = (float)Av.x; m.Av[1] = (float)Av.y; m.Av[2] = (float)Av.z;
+    m.av = (float)((-a3v * nS - K * dot(g.V, g.S)) / vv);
+    m.W[0] = (float)n.x; m.W[1] = (float)n.y; m.W[2] = (float)n.z;
+    m.w0 = (float)(-nS);
+    m.K = (float)K;
+    const d3 L = g.D - g.S;
+    const double ln = sqrt(dot(L, L));
+    m.Zc[0] = (float)(L.x / ln); m.Zc[1] = (float)(L.y / ln); m.Zc[2] = (float)(L.z / ln);
+    m.z0 = (float)(-dot(g.S, L) / ln);
+    m.Lnorm = (float)ln;
+    m.S[0] = (float)g.S.x; m.S[1] = (float)g.S.y; m.S[2] = (float)g.S.z;
+    m.D[0] = (float)g.D.x; m.D[1] = (float)g.D.y; m.D[2] = (float)g.D.z;
+    m.U[0] = (float)g.U.x; m.U[1] = (float)g.U.y; m.U[2] = (float)g.U.z;
+    m.Vv[0] = (float)g.V.x; m.Vv[1] = (float)g.V.y; m.Vv[2] = (float)g.V.z;
+    m.nu = (float)g.nu;
+    m.nv = (float)g.nv;
+    m.uvnorm = (float)(sqrt(uu) * sqrt(vv));
+}
+
+struct BpCfg {
+    bool exact, all1s, mask, direct, bilinear, use_mult;
+    float mask_r2, sep, zval;
+    int zl, zv;
+    uint32_t ssx, ssy, ssz;
+    float w;              // voxwid
+    float c000[3];        // centre of voxel (0,0,0) of the whole volume
+    uint32_t off[3], shp[3];
+    uint32_t nu, nv;      // sinogram shape
+    int64_t s0, s1, s2;   // sinogram strides
+};
+
+__device__ __forceinline__ float powi(float b, int e) {
+    float r = 1.f;
+    if (e < 0) for (int i = 0; i < -e; i++) r /= b;
+    else for (int i = 0; i < e; i++) r *= b;
+    return r;
+}
+
+template <typename Tp>
+__device__ __forceinline__ float det_read(const Tp* __restrict__ img, const BpCfg& c, int j, int k) {
+    if (j < 0 || k < 0 || j >= (int)c.nu || k >= (int)c.nv) return 0.f;
+    return to_f32(img[(int64_t)j * c.s1 + (int64_t)k * c.s2]);
+}
+
+// length of the ray from S through P (unbounded beyond P) inside the cube of
+// side w centred on C, CB_backproject.cu:96-131
+__device__ __forceinline__ float ray_cube_len(const float S[3], const float P[3],
+                                              const float C[3], float w) {
+    float t0 = 0.f, t1 = AP_BIG, d2 = 0.f;
+    for (int a = 0; a < 3; a++) {
+        const float d = P[a] - S[a];
+        d2 += d * d;
+        const float lo = C[a] - 0.5f * w - S[a], hi = C[a] + 0.5f * w - S[a];
+        if (d == 0.f) {
+            if (lo > 0.f || hi < 0.f) return 0.f;
+            continue;
+        }
+        const float inv = 1.f / d;
+        const float ta = lo * inv, tb = hi * inv;
+        t0 = fmaxf(t0, fminf(ta, tb));
+        t1 = fminf(t1, fmaxf(ta, tb));
+    }
+    return t1 > t0 ? (t1 - t0) * sqrtf(d2) : 0.f;
+}
+
+template <typename Tp>
+__device__ float bp_one_view(const BpCfg& c, const ViewMap& m, const Tp* __restrict__ img,
+                             const float x[3]) {
+    const float wx = m.W[0] * x[0] + m.W[1] * x[1] + m.W[2] * x[2] + m.w0;
+    const float iw = 1.f / wx;
+    const float cu = (m.Au[0] * x[0] + m.Au[1] * x[1] + m.Au[2] * x[2] + m.au) * iw;
+    const float cv = (m.Av[0] * x[0] + m.Av[1] * x[1] + m.Av[2] * x[2] + m.av) * iw;
+    float val = 0.f;
+    if (c.exact) {
+        float mnu = cu, mxu = cu, mnv = cv, mxv = cv;
+        for (int q = 0; q < 8; q++) {
+            const float y0 = x[0] + ((q & 4) ? 0.5f : -0.5f) * c.w;
+            const float y1 = x[1] + ((q & 2) ? 0.5f : -0.5f) * c.w;
+            const float y2 = x[2] + ((q & 1) ? 0.5f : -0.5f) * c.w;
+            const float iw2 = 1.f / (m.W[0] * y0 + m.W[1] * y1 + m.W[2] * y2 + m.w0);
+            const float a = (m.Au[0] * y0 + m.Au[1] * y1 + m.Au[2] * y2 + m.au) * iw2;
+            const float b = (m.Av[0] * y0 + m.Av[1] * y1 + m.Av[2] * y2 + m.av) * iw2;
+            mnu = fminf(mnu, a); mxu = fmaxf(mxu, a);
+            mnv = fminf(mnv, b); mxv = fmaxf(mxv, b);
+        }
+        const int j0 = max(0, (int)floorf(mnu + 0.5f * m.nu));
+        const int k0 = max(0, (int)floorf(mnv + 0.5f * m.nv));
+        const int j1 = min((int)c.nu - 1, (int)ceilf(mxu + 0.5f * m.nu));
+        const int k1 = min((int)c.nv - 1, (int)ceilf(mxv + 0.5f * m.nv));
+        for (int j = j0; j <= j1; j++) {
+            const float a = (float)j - 0.5f * (m.nu - 1.f);
+            for (int k = k0; k <= k1; k++) {
+                const float b = (float)k - 0.5f * (m.nv - 1.f);
+                const float P[3] = {m.D[0] + a * m.U[0] + b * m.Vv[0],
+                                    m.D[1] + a * m.U[1] + b * m.Vv[1],
+                                    m.D[2] + a * m.U[2] + b * m.Vv[2]};
+                const float len = ray_cube_len(m.S, P, x, c.w);
+                if (len != 0.f) {
+                    const float y = c.all1s ? 1.f : to_f32(img[(int64_t)j * c.s1 + (int64_t)k * c.s2]);
+                    val += y * len;
+                }
+            }
+        }
+        if (c.zl != 0) {
+            const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
+            val *= powi(zz / m.Lnorm, c.zl);
+        }
+        if (c.zv != 0) {
+            const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
+            val *= powi(zz / c.zval, c.zv);
+        }
+    } else {
+        float mult;
+        if (c.direct) {
+            mult = 1.f;
+            if (c.zl != 0 || c.zv != 0) {
+                const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
+                if (c.zl != 0) mult *= powi(zz / m.Lnorm, c.zl);
+                if (c.zv != 0) mult *= powi(zz / c.zval, c.zv);
+            }
+        } else {
+            // (|det-src| / |vox-src| * voxwid)^2 / (|u||v|) * voxwid
+            const float lam = m.K * iw;
+            mult = lam * lam * c.w * c.w / m.uvnorm;
+            if (c.zl != 0 || c.zv != 0) {
+                const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
+                if (c.zl != 0) mult *= powi(zz / m.Lnorm, c.zl);
+                if (c.zv != 0) mult *= powi(zz / c.zval, c.zv);
+            }
+            mult *= c.w;
+        }
+        if (mult != 0.f) {
+            const float fj = cu + 0.5f * (m.nu - 1.f), fk = cv + 0.5f * (m.nv - 1.f);
+            float y;
+            if (c.bilinear) {
+                if (fj <= -1.f || fk <= -1.f || fj >= (float)c.nu || fk >= (float)c.nv) {
+                    y = 0.f;
+                } else {
+                    const float fl_j = floorf(fj), fl_k = floorf(fk);
+                    const int j0 = (int)fl_j, k0 = (int)fl_k;
+                    const float a = fj - fl_j, b = fk - fl_k;
+                    const float t1 = det_read(img, c, j0, k0) * (1.f - a) + det_read(img, c, j0 + 1, k0) * a;
+                    const float t2 = det_read(img, c, j0, k0 + 1) * (1.f - a) + det_read(img, c, j0 + 1, k0 + 1) * a;
+                    y = t1 * (1.f - b) + t2 * b;
+                }
+            } else {
+                y = det_read(img, c, (int)floorf(fj + 0.5f), (int)floorf(fk + 0.5f));
+            }
+            val = y * mult;
+        }
+    }
+    if (c.sep > 0.f && val != 0.f) {
+        float ud = fabsf(cu) / (0.5f * m.nu), vd = fabsf(cv) / (0.5f * m.nv);
+        ud = fmaxf(0.f, fminf(1.f, (1.f - ud) / c.sep));
+        vd = fmaxf(0.f, fminf(1.f, (1.f - vd) / c.sep));
+        const float pi = 3.14159265358979f;
+        ud = fmaxf(0.f, fminf(1.f, 0.5f * (1.f + cosf(pi * (1.f - ud)))));
+        vd = fmaxf(0.f, fminf(1.f, 0.5f * (1.f + cosf(pi * (1.f - vd)))));
+        val *= ud * vd;
+    }
+    return val;
+}
+
+template <typename Tv, typename Tp, typename Tm>
+__global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
+    BpCfg c, const ap_proj_spec* __restrict__ specs, uint32_t num_projs,
+    Tv* __restrict__ vol, int64_t vs0, int64_t vs1, int64_t vs2,
+    const Tm* __restrict__ mult, int64_t ms0, int64_t ms1, int64_t ms2,
+    const Tp* __restrict__ sino) {
+    __shared__ ViewMap maps[kViewBatch];
+    const uint32_t ck = blockIdx.x * kBz + threadIdx.x;
+    const uint32_t cj = blockIdx.y * kBy + threadIdx.y;
+    const uint32_t ci = blockIdx.z;
+    const bool in = ck < c.shp[2] && cj < c.shp[1] && ci < c.shp[0];
+    const uint32_t gi = ci + c.off[0], gj = cj + c.off[1], gk = ck + c.off[2];
+    const float xc[3] = {c.c000[0] + (float)gi * c.w, c.c000[1] + (float)gj * c.w,
+                         c.c000[2] + (float)gk * c.w};
+    bool active = in;
+    if (c.mask && xc[0] * xc[0] + xc[1] * xc[1] > c.mask_r2) active = false;
+    const uint32_t nss = c.ssx * c.ssy * c.ssz;
+    // sub-voxel accumulators live in registers for the common ss=1 case
+    float acc1 = 0.f;
+    float ax = 0.f;
+    const int tid = threadIdx.y * kBz + threadIdx.x;
+    for (uint32_t p0 = 0; p0 < num_projs; p0 += kViewBatch) {
+        const uint32_t nb = min((uint32_t)kViewBatch, num_projs - p0);
+        __syncthreads();
+        if (tid < (int)nb) make_view_map(specs, p0 + tid, maps[tid]);
+        __syncthreads();
+        if (!active) continue;
+        if (nss == 1) {
+            for (uint32_t q = 0; q < nb; q++)
+                acc1 += bp_one_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
+        }
+    }
+    if (nss != 1 && active) {
+        // general sub-voxel sampling: views re-derived per sub-voxel
+        // (CB_backproject.cu:377-383 loop order: x, y, z outermost to innermost)
+        ax = 0.f;
+        for (uint32_t si = 0; si < c.ssx; si++) {
+            float ay = 0.f;
+            for (uint32_t sj = 0; sj < c.ssy; sj++) {
+                float az = 0.f;
+                for (uint32_t sk = 0; sk < c.ssz; sk++) {
+                    const float xs[3] = {xc[0] + (-0.5f + (0.5f + si) / c.ssx) * c.w,
+                                         xc[1] + (-0.5f + (0.5f + sj) / c.ssy) * c.w,
+                                         xc[2] + (-0.5f + (0.5f + sk) / c.ssz) * c.w};
+                    float acc = 0.f;
+                    for (uint32_t p = 0; p < num_projs; p++) {
+                        ViewMap m;
+                        make_view_map(specs, p, m);
+                        acc += bp_one_view(c, m, sino + (int64_t)p * c.s0, xs);
+                    }
+                    az += acc;
+                }
+                ay += az / c.ssz;
+            }
+            ax += ay / c.ssy;
+        }
+        acc1 = ax / c.ssx;
+    }
+    if (!active || acc1 == 0.f) return;
+    float mval = 1.f;
+    if (c.use_mult) mval = to_f32(mult[(int64_t)gi * ms0 + (int64_t)gj * ms1 + (int64_t)gk * ms2]);
+    const int64_t o = (int64_t)gi * vs0 + (int64_t)gj * vs1 + (int64_t)gk * vs2;
+    vol[o] = from_f32<Tv>(to_f32(vol[o]) + acc1 * mval);
+}
+
+}  // namespace
+
+int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_proj_spec* specs,
+                                 uint32_t num_projs, const ap_vol_tex_chunk_spec* chunk,
+                                 const ap_array3* vol_desc, void* vol,
+                                 const ap_array3* mult_desc, const void* multiplier,
+                                 const ap_array3* sino_desc, const void* sino, hipStream_t s) {
+    BpCfg c;
+    c.exact = cfg->exact_line_integ_backproj != 0;
+    c.all1s = cfg->backproject_all_1s != 0;
+    c.mask = cfg->mask_to_cylinder != 0;
+    c.mask_r2 = cfg->mask_to_cyl_rad * cfg->mask_to_cyl_rad;
+    c.direct = cfg->approx_mode_direct_raycast != 0;
+    c.bilinear = cfg->approx_mode_linear_interpolate != 0;
+    c.use_mult = cfg->apply_summand_multiplier != 0 && multiplier != nullptr;
+    c.sep = cfg->soften_edge_proportion;
+    c.zl = cfg->Z_on_L_factor_power;
+    c.zv = cfg->Z_on_val_factor_power;
+    c.zval = cfg->Z_on_val_factor_val;
+    c.ssx = cfg->ssx; c.ssy = cfg->ssy; c.ssz = cfg->ssz;
+    c.w = chunk->voxwid;
+    c.c000[0] = chunk->texpos.x + 0.5f * chunk->voxwid;
+    c.c000[1] = chunk->texpos.y + 0.5f * chunk->voxwid;
+    c.c000[2] = chunk->texpos.z + 0.5f * chunk->voxwid;
+    c.off[0] = chunk->offsets.x; c.off[1] = chunk->offsets.y; c.off[2] = chunk->offsets.z;
+    c.shp[0] = chunk->shape.x; c.shp[1] = chunk->shape.y; c.shp[2] = chunk->shape.z;
+    c.nu = sino_desc->shape[1];
+    c.nv = sino_desc->shape[2];
+    c.s0 = sino_desc->strides[0]; c.s1 = sino_desc->strides[1]; c.s2 = sino_desc->strides[2];
+    dim3 block(kBz, kBy, 1);
+    dim3 grid((c.shp[2] + kBz - 1) / kBz, (c.shp[1] + kBy - 1) / kBy, c.shp[0]);
+    const int32_t mdt = multiplier ? mult_desc->dtype : vol_desc->dtype;
+    const int64_t* ms = multiplier ? mult_desc->strides : vol_desc->strides;
+    return dispatch_dtype(vol_desc->dtype, [&](auto tv) -> int32_t {
+        using Tv = decltype(tv);
+        return dispatch_dtype(sino_desc->dtype, [&](auto tp) -> int32_t {
+            using Tp = decltype(tp);
+            return dispatch_dtype(mdt, [&](auto tm) -> int32_t {
+                using Tm = decltype(tm);
+                if constexpr (std::is_same<Tv, double>::value || std::is_same<Tp, double>::value ||
+                              std::is_same<Tm, double>::value || std::is_same<Tp, bf16_t>::value) {
+                    return ap_fail(AP_ERR_UNSUPPORTED, "unsupported dtype combination");
+                } else {
+                    hipLaunchKernelGGL((k_backproject_voxel<Tv, Tp, Tm>), grid, block, 0, s, c, specs,
+                                       num_projs, (Tv*)vol, vol_desc->strides[0], vol_desc->strides[1],
+                                       vol_desc->strides[2], (const Tm*)multiplier, ms[0], ms[1], ms[2],
+                                       (const Tp*)sino);
+                    return ap_check_launch("backproject_voxel");
+                }
+            });
+        });
+    });
+}
+
+extern "C" int32_t ap_backproject(const ap_config_cb_backproject* cfg, const ap_proj_spec* specs,
+                                  uint32_t num_projs, const ap_vol_tex_chunk_spec* chunk,
+                                  const ap_array3* vol_desc, void* vol,
+                                  const ap_array3* mult_desc, const void* multiplier,
+                                  const ap_array3* sino_desc, const void* sino, void* stream) {
+    AP_REQUIRE(cfg && chunk && specs, "NULL config, chunk or specs");
+    int32_t st = ap_check_array3(vol_desc, vol, "volume", true, true, false);
+    if (st) return st;
+    st = ap_check_array3(sino_desc, sino, "sino", true, false, false);
+    if (st) return st;
+    AP_REQUIRE(sino_desc->shape[0] == num_projs, "sino has %u views, num_projs=%u",
+               sino_desc->shape[0], num_projs);
+    if (cfg->backproj_weighting_preset != 0)
+        return ap_fail(AP_ERR_UNSUPPORTED, "backproj_weighting_preset %u (GBC) is not supported",
+                       (unsigned)cfg->backproj_weighting_preset);
+    AP_REQUIRE(cfg->ssx >= 1 && cfg->ssy >= 1 && cfg->ssz >= 1, "ssx/ssy/ssz must be >= 1");
+    AP_REQUIRE(chunk->voxwid > 0.f, "voxwid must be positive");
+    if ((cfg->apply_summand_multiplier != 0) != (multiplier != nullptr))
+        return ap_fail(AP_ERR_INVALID,
+                       "apply_summand_multiplier must be set if and only if a multiplier is given");
+    if (multiplier) {
+        if ((st = ap_check_array3(mult_desc, multiplier, "multiplier", true, true, false))) return st;
+        for (int a = 0; a < 3; a++)
+            AP_REQUIRE(mult_desc->shape[a] == vol_desc->shape[a], "multiplier shape != volume shape");
+    }
+    const uint32_t off[3] = {chunk->offsets.x, chunk->offsets.y, chunk->offsets.z};
+    const uint32_t shp[3] = {chunk->shape.x, chunk->shape.y, chunk->shape.z};
+    for (int a = 0; a < 3; a++)
+        AP_REQUIRE((uint64_t)off[a] + shp[a] <= vol_desc->shape[a], "chunk exceeds the volume");
+    if (num_projs == 0 || shp[0] == 0 || shp[1] == 0 || shp[2] == 0) return AP_OK;
+    AP_REQUIRE(shp[0] <= 65535u, "chunk x extent above 65535");
+    return launch_backproject_voxel(cfg, specs, num_projs, chunk, vol_desc, vol, mult_desc,
+                                    multiplier, sino_desc, sino, (hipStream_t)stream);
+}

@@ -1,0 +1,496 @@
+// forbild.hip -- analytic FORBILD primitives on the GPU: exact last-wins line
+// integrals (ap_project_forbild), point-sampling voxeliser
+// (ap_voxelise_forbild) and per-shape bounding boxes (ap_forbild_bboxes).
+//
+// Every primitive is reduced once per block to an affine map M (in FP64)
+// sending it onto a unit reference shape -- the cube [-1,1]^3, the unit ball,
+// or the unit-radius cylinder |z| <= h -- so a ray needs one 3x3 transform and
+// one quadratic or slab solve per shape.  Ray-primitive parameters are FP64
+// (grazing rays cancel catastrophically in FP32); the compositing sweep and
+// accumulation are FP32/FP64 as the output dtype requires.
+//
+// Semantics: cuda/structs/cu/FORBILD_{Box,Sphere,Ellipsoid,Ellipsoid_free,
+// Cylinder,Cylinder_x,Cylinder_y,Cylinder_z,Ellipt_Cyl,Ellipt_Cyl_x,
+// Ellipt_Cyl_y,Ellipt_Cyl_z,ClipAxisSpec}.cu (intervals / membership),
+// FORBILD_Shape.cu:474-540 (last shape wins), CB_project_texture_chunked.cu
+// :421-593 (sub-pixels, multiply_before_adding, overflow counter).
+#include "ap_common.h"
+
+namespace {
+
+enum PrimClass : int { kNone = 0, kCube = 1, kBall = 2, kCyl = 3 };
+
+struct Prim {
+    double M[9];      // rows of the map r -> M (r - c)
+    double c[3];
+    double h;         // cylinder half length
+    float clip_ax[3][3];
+    float clip_d[3];
+    int clip_mode[3];
+    float rho;
+    int cls;
+};
+
+__device__ __forceinline__ d3 unit_of(ap_fvec3 a) {
+    d3 v = fv(a);
+    return v * (1.0 / sqrt(dot(v, v)));
+}
+
+__device__ void make_prim(const ap_forbild_shape& s, Prim& P) {
+    d3 ex = mkd3(1, 0, 0), ey = mkd3(0, 1, 0), ez = mkd3(0, 0, 1);
+    double sx = 1, sy = 1, sz = 1;
+    P.h = 0;
+    P.cls = kNone;
+    switch (s.type) {
+    case 1: P.cls = kCube; sx = 2.0 / s.dx; sy = 2.0 / s.dy; sz = 2.0 / s.dz; break;
+    case 10: P.cls = kBall; sx = 1.0 / s.dx; sy = 1.0 / s.dy; sz = 1.0 / s.dz; break;
+    case 11:
+        P.cls = kBall;
+        ex = unit_of(s.a_x); ey = unit_of(s.a_y); ez = unit_of(s.a_z);
+        sx = 1.0 / s.dx; sy = 1.0 / s.dy; sz = 1.0 / s.dz;
+        break;
+    case 13: P.cls = kBall; sx = sy = sz = 1.0 / s.r; break;
+    case 6: case 7: case 8: case 9: {
+        P.cls = kCyl;
+        d3 ax = s.type == 6 ? unit_of(s.axis)
+              : s.type == 7 ? mkd3(1, 0, 0) : s.type == 8 ? mkd3(0, 1, 0) : mkd3(0, 0, 1);
+        d3 t = fabs(ax.x) < 0.9 ? mkd3(1, 0, 0) : mkd3(0, 1, 0);
+        d3 e1 = cross(ax, t);
+        e1 = e1 * (1.0 / sqrt(dot(e1, e1)));
+        ex = e1; ey = cross(ax, e1); ez = ax;
+        sx = sy = 1.0 / s.r; sz = 1.0;
+        P.h = 0.5 * (double)s.l;
+        break;
+    }
+    case 12: case 14: case 15: case 16:
+        P.cls = kCyl;
+        if (s.type == 12) { ex = unit_of(s.a_x); ey = unit_of(s.a_y); ez = unit_of(s.axis); }
+        else if (s.type == 14) { ex = mkd3(0, 1, 0); ey = mkd3(0, 0, 1); ez = mkd3(1, 0, 0); }
+        else if (s.type == 15) { ex = mkd3(0, 0, 1); ey = mkd3(1, 0, 0); ez = mkd3(0, 1, 0); }
+        sx = 1.0 / s.dx; sy = 1.0 / s.dy; sz = 1.0;
+        P.h = 0.5 * (double)s.l;
+        break;
+    default: break;
+    }
+    P.M[0] = ex.x * sx; P.M[1] = ex.y * sx; P.M[2] = ex.z * sx;
+    P.M[3] = ey.x * sy; P.M[4] = ey.y * sy; P.M[5] = ey.z * sy;
+    P.M[6] = ez.x * sz; P.M[7] = ez.y * sz; P.M[8] = ez.z * sz;
+    P.c[0] = s.x; P.c[1] = s.y; P.c[2] = s.z;
+    const ap_forbild_clip* cs[3] = {&s.cas1, &s.cas2, &s.cas3};
+    for (int i = 0; i < 3; i++) {
+        P.clip_ax[i][0] = cs[i]->axis.x;
+        P.clip_ax[i][1] = cs[i]->axis.y;
+        P.clip_ax[i][2] = cs[i]->axis.z;
+        P.clip_d[i] = cs[i]->dotprod;
+        P.clip_mode[i] = cs[i]->mode;
+    }
+    P.rho = s.rho;
+}
+
+// stable roots of A t^2 + 2 B t + C = 0; false if no real interval
+__device__ __forceinline__ bool quad_roots(double A, double B, double C, double& t0, double& t1) {
+    if (A == 0.0) return false;
+    const double disc = B * B - A * C;
+    if (!(disc > 0.0)) return false;
+    const double sq = sqrt(disc);
+    const double q = B > 0.0 ? -(B + sq) : -(B - sq);
+    const double r1 = q / A, r2 = q != 0.0 ? C / q : -r1;
+    t0 = fmin(r1, r2);
+    t1 = fmax(r1, r2);
+    return true;
+}
+
+// Parametric interval (within [0,1]) of S + t d inside the primitive
+__device__ bool prim_interval(const Prim& P, const d3& S, const d3& d, double& ta, double& tb) {
+    ta = 0.0;
+    tb = 1.0;
+    for (int i = 0; i < 3; i++) {
+        if (P.clip_mode[i] == 0) continue;
+        const double ax = P.clip_ax[i][0], ay = P.clip_ax[i][1], az = P.clip_ax[i][2];
+        double s0 = ax * S.x + ay * S.y + az * S.z - (double)P.clip_d[i];
+        double sd = ax * d.x + ay * d.y + az * d.z;
+        if (P.clip_mode[i] == 2) { s0 = -s0; sd = -sd; }   // keep s0 + t sd <= 0
+        if (sd == 0.0) {
+            if (s0 > 0.0) return false;
+        } else {
+            const double t = -s0 / sd;
+            if (sd > 0.0) tb = fmin(tb, t); else ta = fmax(ta, t);
+        }
+    }
+    if (!(tb > ta)) return false;
+    const double r0 = S.x - P.c[0], r1 = S.y - P.c[1], r2 = S.z - P.c[2];
+    const double ls[3] = {P.M[0] * r0 + P.M[1] * r1 + P.M[2] * r2,
+                          P.M[3] * r0 + P.M[4] * r1 + P.M[5] * r2,
+                          P.M[6] * r0 + P.M[7] * r1 + P.M[8] * r2};
+    const double ld[3] = {P.M[0] * d.x + P.M[1] * d.y + P.M[2] * d.z,
+                          P.M[3] * d.x + P.M[4] * d.y + P.M[5] * d.z,
+                          P.M[6] * d.x + P.M[7] * d.y + P.M[8] * d.z};
+    double t0, t1;
+    switch (P.cls) {
+    case kCube:
+        for (int a = 0; a < 3; a++) {
+            if (ld[a] == 0.0) {
+                if (fabs(ls[a]) > 1.0) return false;
+                continue;
+            }
+            const double inv = 1.0 / ld[a];
+            const double x0 = (-1.0 - ls[a]) * inv, x1 = (1.0 - ls[a]) * inv;
+            ta = fmax(ta, fmin(x0, x1));
+            tb = fmin(tb, fmax(x0, x1));
+        }
+        break;
+    case kBall:
+        if (!quad_roots(ld[0] * ld[0] + ld[1] * ld[1] + ld[2] * ld[2],
+                        ls[0] * ld[0] + ls[1] * ld[1] + ls[2] * ld[2],
+                        ls[0] * ls[0] + ls[1] * ls[1] + ls[2] * ls[2] - 1.0, t0, t1))
+            return false;
+        ta = fmax(ta, t0);
+        tb = fmin(tb, t1);
+        break;
+    case kCyl: {
+        if (ld[2] == 0.0) {
+            if (fabs(ls[2]) > P.h) return false;
+        } else {
+            const double inv = 1.0 / ld[2];
+            const double x0 = (-P.h - ls[2]) * inv, x1 = (P.h - ls[2]) * inv;
+            ta = fmax(ta, fmin(x0, x1));
+            tb = fmin(tb, fmax(x0, x1));
+        }
+        const double A = ld[0] * ld[0] + ld[1] * ld[1];
+        const double C = ls[0] * ls[0] + ls[1] * ls[1] - 1.0;
+        if (A == 0.0) {
+            if (C > 0.0) return false;
+        } else {
+            if (!quad_roots(A, ls[0] * ld[0] + ls[1] * ld[1], C, t0, t1)) return false;
+            ta = fmax(ta, t0);
+            tb = fmin(tb, t1);
+        }
+        break;
+    }
+    default:
+        return false;
+    }
+    return tb > ta;
+}
+
+__device__ bool prim_contains(const Prim& P, const d3& p) {
+    for (int i = 0; i < 3; i++) {
+        if (P.clip_mode[i] == 0) continue;
+        const double dp = (double)P.clip_ax[i][0] * p.x + (double)P.clip_ax[i][1] * p.y +
+                          (double)P.clip_ax[i][2] * p.z;
+        if (P.clip_mode[i] == 1 && !(dp <= (double)P.clip_d[i])) return false;
+        if (P.clip_mode[i] == 2 && !(dp > (double)P.clip_d[i])) return false;
+    }
+    const double r0 = p.x - P.c[0], r1 = p.y - P.c[1], r2 = p.z - P.c[2];
+    const double a = P.M[0] * r0 + P.M[1] * r1 + P.M[2] * r2;
+    const double b = P.M[3] * r0 + P.M[4] * r1 + P.M[5] * r2;
+    const double e = P.M[6] * r0 + P.M[7] * r1 + P.M[8] * r2;
+    switch (P.cls) {
+    case kCube: return fabs(a) <= 1.0 && fabs(b) <= 1.0 && fabs(e) <= 1.0;
+    case kBall: return a * a + b * b + e * e <= 1.0;
+    case kCyl: return fabs(e) <= P.h && a * a + b * b <= 1.0;
+    default: return false;
+    }
+}
+
+constexpr int kProjThreads = 64;   // one wave per block
+constexpr int kShapeBatch = 64;    // shapes staged in LDS at a time
+constexpr int kSlots = 24;         // stored intervals per ray before the exact fallback
+constexpr uint32_t kRefMaxIntersections = 1000;   // reference's overflow threshold
+
+struct Slot { double a, b; float rho; int k; };
+
+// last-wins integral over stored intervals (the highest shape index k that
+// covers a point decides its density)
+__device__ double sweep_last_wins(const Slot* s, int m) {
+    double acc = 0.0;
+    double prog = -1.0;
+    for (int guard = 0; guard < 4 * m + 4; guard++) {
+        // top interval covering prog (or the next start when none covers)
+        int top = -1;
+        double nxt = 2.0;
+        for (int i = 0; i < m; i++) {
+            if (s[i].a <= prog && prog < s[i].b) {
+                if (top < 0 || s[i].k > s[top].k) top = i;
+            }
+        }
+        if (top >= 0) {
+            nxt = s[top].b;
+            for (int i = 0; i < m; i++)
+                if (s[i].k > s[top].k && s[i].a > prog && s[i].a < nxt) nxt = s[i].a;
+            acc += (nxt - prog) * (double)s[top].rho;
+            prog = nxt;
+        } else {
+            for (int i = 0; i < m; i++)
+                if (s[i].a > prog && s[i].a < nxt) nxt = s[i].a;
+            if (nxt >= 2.0) break;
+            prog = nxt;
+        }
+    }
+    return acc;
+}
+
+template <typename Tp>
+__global__ __launch_bounds__(kProjThreads) void k_project_forbild(
+    float mba, int additive, uint32_t ssu, uint32_t ssv, const ap_proj_spec* __restrict__ specs,
+    const ap_forbild_shape* __restrict__ shapes, uint32_t K, uint32_t nu, uint32_t nv,
+    int64_t s0, int64_t s1, int64_t s2, Tp* __restrict__ sino, uint32_t* __restrict__ oom) {
+    __shared__ Prim prims[kShapeBatch];
+    __shared__ Slot slots[kProjThreads][kSlots];
+    const uint32_t pix = blockIdx.x * kProjThreads + threadIdx.x;
+    const uint32_t p = blockIdx.y;
+    const bool in = pix < nu * nv;
+    const uint32_t j = in ? pix / nv : 0, k = in ? pix % nv : 0;
+    const ViewGeom g = load_view(specs, p);
+    double tot = 0.0;
+    bool overflow = false;
+    for (uint32_t su = 0; su < ssu; su++) {
+        double accu = 0.0;
+        for (uint32_t sv = 0; sv < ssv; sv++) {
+            const double ju = (double)j - 0.5 + ((double)su + 0.5) / ssu;
+            const double kv = (double)k - 0.5 + ((double)sv + 0.5) / ssv;
+            const double a = ju - 0.5 * (double)(g.nu - 1), b = kv - 0.5 * (double)(g.nv - 1);
+            const d3 Q = g.D + g.U * a + g.V * b;
+            const d3 d = Q - g.S;
+            const double len = sqrt(dot(d, d));
+            int m = 0;
+            uint32_t nhit = 0;
+            bool spill = false;
+            double val = 0.0;
+            for (uint32_t k0 = 0; k0 < K; k0 += kShapeBatch) {
+                const uint32_t nb = min((uint32_t)kShapeBatch, K - k0);
+                __syncthreads();
+                if (threadIdx.x < nb) make_prim(shapes[k0 + threadIdx.x], prims[threadIdx.x]);
+                __syncthreads();
+                if (!in || len == 0.0) continue;
+                for (uint32_t q = 0; q < nb; q++) {
+                    double ta, tb;
+                    if (!prim_interval(prims[q], g.S, d, ta, tb)) continue;
+                    nhit++;
+                    if (nhit > kRefMaxIntersections) { overflow = true; continue; }
+                    if (additive) { val += (tb - ta) * (double)prims[q].rho; continue; }
+                    if (m < kSlots) {
+                        slots[threadIdx.x][m] = Slot{ta, tb, prims[q].rho, (int)(k0 + q)};
+                        m++;
+                    } else {
+                        spill = true;
+                    }
+                }
+            }
+            if (in && len > 0.0 && !additive) {
+                if (!spill) {
+                    val = sweep_last_wins(slots[threadIdx.x], m);
+                } else {
+                    // more intersecting shapes than slots: exact sweep that
+                    // re-derives intervals from global memory at every event
+                    double prog = -1.0;
+                    for (uint32_t guard = 0; guard < 4 * nhit + 4; guard++) {
+                        int top = -1;
+                        double topb = 0.0, nxt = 2.0;
+                        float toprho = 0.f;
+                        for (int q = (int)K - 1; q >= 0; q--) {
+                            Prim P;
+                            make_prim(shapes[q], P);
+                            double ta, tb;
+                            if (!prim_interval(P, g.S, d, ta, tb)) continue;
+                            if (top < 0 && ta <= prog && prog < tb) {
+                                top = q; topb = tb; toprho = P.rho;
+                            } else if (top < 0 && ta > prog && ta < nxt) {
+                                nxt = ta;   // candidate next start (no cover yet)
+                            } else if (top >= 0 && ta > prog) {
+                                // shapes below the top cannot interrupt it
+                            }
+                            (void)topb;
+                        }
+                        if (top >= 0) {
+                            double e = topb;
+                            for (int q = (int)K - 1; q > top; q--) {
+                                Prim P;
+                                make_prim(shapes[q], P);
+                                double ta, tb;
+                                if (prim_interval(P, g.S, d, ta, tb) && ta > prog && ta < e) e = ta;
+                            }
+                            val += (e - prog) * (double)toprho;
+                            prog = e;
+                        } else {
+                            if (nxt >= 2.0) break;
+                            prog = nxt;
+                        }
+                    }
+                }
+            }
+            accu += val * len;
+        }
+        tot += accu / ssv;
+    }
+    tot /= ssu;
+    if (!in) return;
+    if (overflow && oom) atomicAdd(oom, 1u);
+    const int64_t o = (int64_t)p * s0 + (int64_t)j * s1 + (int64_t)k * s2;
+    sino[o] = (Tp)((double)sino[o] + (double)mba * tot);
+}
+
+template <typename Tv>
+__global__ __launch_bounds__(256) void k_voxelise(
+    const ap_forbild_shape* __restrict__ shapes, uint32_t K, double vw, double tpx, double tpy,
+    double tpz, uint32_t ox, uint32_t oy, uint32_t oz, uint32_t cx, uint32_t cy, uint32_t cz,
+    Tv* __restrict__ vol, int64_t v0, int64_t v1, int64_t v2, uint32_t ss) {
+    __shared__ Prim prims[kShapeBatch];
+    const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+    const uint32_t j = blockIdx.y * 4 + threadIdx.y;
+    const uint32_t i = blockIdx.z;
+    const bool in = k < cz && j < cy && i < cx;
+    const uint32_t gi = i + ox, gj = j + oy, gk = k + oz;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const uint32_t npt = ss * ss * ss;
+    // per sub-point value of the last containing shape; ss <= 4 => <= 64 points
+    float val[64];
+    for (uint32_t q = 0; q < npt && q < 64; q++) val[q] = 0.f;
+    for (uint32_t k0 = 0; k0 < K; k0 += kShapeBatch) {
+        const uint32_t nb = min((uint32_t)kShapeBatch, K - k0);
+        __syncthreads();
+        if (tid < (int)nb) make_prim(shapes[k0 + tid], prims[tid]);
+        __syncthreads();
+        if (!in) continue;
+        for (uint32_t a = 0, q = 0; a < ss; a++)
+            for (uint32_t b = 0; b < ss; b++)
+                for (uint32_t e = 0; e < ss; e++, q++) {
+                    const double fx = ((double)a - 0.5 * (ss - 1.0)) / ss;
+                    const double fy = ((double)b - 0.5 * (ss - 1.0)) / ss;
+                    const double fz = ((double)e - 0.5 * (ss - 1.0)) / ss;
+                    const d3 pnt = mkd3(tpx + (gi + 0.5 + fx) * vw, tpy + (gj + 0.5 + fy) * vw,
+                                        tpz + (gk + 0.5 + fz) * vw);
+                    for (uint32_t s = 0; s < nb; s++)
+                        if (prim_contains(prims[s], pnt)) val[q] = prims[s].rho;
+                }
+    }
+    if (!in) return;
+    float tot = 0.f;
+    for (uint32_t q = 0; q < npt; q++) tot += val[q];
+    tot /= (float)npt;
+    const int64_t o = (int64_t)gi * v0 + (int64_t)gj * v1 + (int64_t)gk * v2;
+    vol[o] = from_f32<Tv>(to_f32(vol[o]) + tot);
+}
+
+__global__ void k_bboxes(const ap_forbild_shape* __restrict__ shapes, uint32_t K, float* out) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= K) return;
+    Prim P;
+    make_prim(shapes[q], P);
+    // extent of the reference shape mapped back: r = c + M^-1 y.  For the
+    // unit reference shapes, half-extent along world axis w is
+    //   cube: sum_a |Minv[w][a]|, ball: |Minv row w|_2,
+    //   cylinder: |(Minv[w][0],Minv[w][1])|_2 + h |Minv[w][2]|.
+    const double* M = P.M;
+    const double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+                       M[2] * (M[3] * M[7] - M[4] * M[6]);
+    double Mi[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (det != 0.0) {
+        const double id = 1.0 / det;
+        Mi[0] = (M[4] * M[8] - M[5] * M[7]) * id;
+        Mi[1] = (M[2] * M[7] - M[1] * M[8]) * id;
+        Mi[2] = (M[1] * M[5] - M[2] * M[4]) * id;
+        Mi[3] = (M[5] * M[6] - M[3] * M[8]) * id;
+        Mi[4] = (M[0] * M[8] - M[2] * M[6]) * id;
+        Mi[5] = (M[2] * M[3] - M[0] * M[5]) * id;
+        Mi[6] = (M[3] * M[7] - M[4] * M[6]) * id;
+        Mi[7] = (M[1] * M[6] - M[0] * M[7]) * id;
+        Mi[8] = (M[0] * M[4] - M[1] * M[3]) * id;
+    }
+    for (int w = 0; w < 3; w++) {
+        const double a = Mi[3 * w], b = Mi[3 * w + 1], e = Mi[3 * w + 2];
+        double h;
+        if (P.cls == kCube) h = fabs(a) + fabs(b) + fabs(e);
+        else if (P.cls == kBall) h = sqrt(a * a + b * b + e * e);
+        else if (P.cls == kCyl) h = sqrt(a * a + b * b) + P.h * fabs(e);
+        else h = 0.0;
+        out[6 * q + w] = (float)(P.c[w] - h);
+        out[6 * q + 3 + w] = (float)(P.c[w] + h);
+    }
+}
+
+int32_t check_shapes(const ap_forbild_shape* shapes, uint32_t K) {
+    if (K > 0 && !shapes) return ap_fail(AP_ERR_INVALID, "shapes is NULL");
+    return AP_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t ap_project_forbild(const ap_config_cb_project_forbild* cfg,
+                                      const ap_proj_spec* specs, uint32_t num_projs,
+                                      const ap_forbild_shape* shapes, uint32_t num_shapes,
+                                      const ap_array3* sino_desc, void* sino,
+                                      uint32_t* out_of_mem_count, void* stream) {
+    AP_REQUIRE(cfg && specs, "NULL config or specs");
+    AP_REQUIRE(cfg->ssu >= 1 && cfg->ssv >= 1, "ssu and ssv must be >= 1");
+    int32_t st = check_shapes(shapes, num_shapes);
+    if (st) return st;
+    if ((st = ap_check_array3(sino_desc, sino, "sino", false, false, true))) return st;
+    AP_REQUIRE(sino_desc->shape[0] == num_projs, "sino has %u views, num_projs=%u",
+               sino_desc->shape[0], num_projs);
+    const uint32_t nu = sino_desc->shape[1], nv = sino_desc->shape[2];
+    if (num_projs == 0 || nu == 0 || nv == 0) return AP_OK;
+    AP_REQUIRE(num_projs <= 65535u, "at most 65535 views per call");
+    const uint64_t npix = (uint64_t)nu * nv;
+    AP_REQUIRE(npix < (1ull << 31), "detector too large");
+    dim3 grid((uint32_t)((npix + kProjThreads - 1) / kProjThreads), num_projs, 1);
+    hipStream_t s = (hipStream_t)stream;
+    if (sino_desc->dtype == AP_F32) {
+        hipLaunchKernelGGL(k_project_forbild<float>, grid, dim3(kProjThreads), 0, s,
+                           cfg->multiply_before_adding, (int)cfg->overlap_addition, cfg->ssu, cfg->ssv,
+                           specs, shapes, num_shapes, nu, nv, sino_desc->strides[0],
+                           sino_desc->strides[1], sino_desc->strides[2], (float*)sino, out_of_mem_count);
+    } else if (sino_desc->dtype == AP_F64) {
+        hipLaunchKernelGGL(k_project_forbild<double>, grid, dim3(kProjThreads), 0, s,
+                           cfg->multiply_before_adding, (int)cfg->overlap_addition, cfg->ssu, cfg->ssv,
+                           specs, shapes, num_shapes, nu, nv, sino_desc->strides[0],
+                           sino_desc->strides[1], sino_desc->strides[2], (double*)sino, out_of_mem_count);
+    } else {
+        return ap_fail(AP_ERR_UNSUPPORTED, "FORBILD sinogram must be F32 or F64");
+    }
+    return ap_check_launch("project_forbild");
+}
+
+extern "C" int32_t ap_voxelise_forbild(const ap_forbild_shape* shapes, uint32_t num_shapes,
+                                       const ap_vol_tex_chunk_spec* chunk,
+                                       const ap_array3* vol_desc, void* vol,
+                                       uint32_t supersample, void* stream) {
+    AP_REQUIRE(chunk, "chunk is NULL");
+    AP_REQUIRE(supersample >= 1 && supersample <= 4, "supersample must be in 1..4");
+    AP_REQUIRE(chunk->voxwid > 0.f, "voxwid must be positive");
+    int32_t st = check_shapes(shapes, num_shapes);
+    if (st) return st;
+    if ((st = ap_check_array3(vol_desc, vol, "volume", true, true, false))) return st;
+    const uint32_t off[3] = {chunk->offsets.x, chunk->offsets.y, chunk->offsets.z};
+    const uint32_t shp[3] = {chunk->shape.x, chunk->shape.y, chunk->shape.z};
+    for (int a = 0; a < 3; a++)
+        AP_REQUIRE((uint64_t)off[a] + shp[a] <= vol_desc->shape[a], "chunk exceeds the volume");
+    if (shp[0] == 0 || shp[1] == 0 || shp[2] == 0) return AP_OK;
+    AP_REQUIRE(shp[0] <= 65535u, "chunk x extent above 65535");
+    dim3 grid((shp[2] + 63) / 64, (shp[1] + 3) / 4, shp[0]);
+    hipStream_t s = (hipStream_t)stream;
+    return dispatch_dtype(vol_desc->dtype, [&](auto tv) -> int32_t {
+        using Tv = decltype(tv);
+        if constexpr (std::is_same<Tv, double>::value) {
+            return ap_fail(AP_ERR_UNSUPPORTED, "float64 volumes are not supported");
+        } else {
+            hipLaunchKernelGGL(k_voxelise<Tv>, grid, dim3(64, 4, 1), 0, s, shapes, num_shapes,
+                               (double)chunk->voxwid, (double)chunk->texpos.x, (double)chunk->texpos.y,
+                               (double)chunk->texpos.z, off[0], off[1], off[2], shp[0], shp[1], shp[2],
+                               (Tv*)vol, vol_desc->strides[0], vol_desc->strides[1],
+                               vol_desc->strides[2], supersample);
+            return ap_check_launch("voxelise_forbild");
+        }
+    });
+}
+
+extern "C" int32_t ap_forbild_bboxes(const ap_forbild_shape* shapes, uint32_t num_shapes,
+                                     float* bboxes, void* stream) {
+    int32_t st = check_shapes(shapes, num_shapes);
+    if (st) return st;
+    AP_REQUIRE(num_shapes == 0 || bboxes, "bboxes is NULL");
+    if (num_shapes == 0) return AP_OK;
+    hipLaunchKernelGGL(k_bboxes, dim3((num_shapes + 127) / 128), dim3(128), 0, (hipStream_t)stream,
+                       shapes, num_shapes, bboxes);
+    return ap_check_launch("forbild_bboxes");
+}

@@ -1,0 +1,106 @@
+"""The C-ABI library: loads without a GPU, exports every declared symbol,
+has the reference's struct layouts, and rejects bad arguments with error
+codes before touching the device (no compute calls here)."""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+import aprict_amd
+from aprict_amd import _abi
+from aprict_amd.cb import flatpanel as cbfp
+
+HEADER = pathlib.Path(__file__).resolve().parents[1] / "include" / "aprict_hip.h"
+
+
+def declared_functions() -> list[str]:
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(ap_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    lib = aprict_amd.lib()
+    names = declared_functions()
+    assert len(names) == len(_abi.SIGNATURES)
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _abi.SIGNATURES, f"{n} has no ctypes signature"
+
+
+def test_struct_layouts_match_reference_sizes():
+    lib = aprict_amd.lib()
+    want = {0: 48, 1: 60, 2: 40, 3: 16, 4: 16, 5: 60, 6: 20, 7: 156, 8: 40}
+    for which, size in want.items():
+        assert lib.ap_struct_size(which) == size
+    assert lib.ap_struct_size(99) == -1
+    assert _abi.forbild_shape.fields["x"][1] == 68
+    assert _abi.proj_spec.fields["dspec"][1] == 12
+    assert _abi.config_cb_backproject.fields["ssx"][1] == 32
+
+
+def _desc(shape, dtype=_abi.AP_F32):
+    d = _abi.Array3()
+    d.dtype = dtype
+    st = [shape[1] * shape[2], shape[2], 1]
+    for a in range(3):
+        d.shape[a] = shape[a]
+        d.strides[a] = st[a]
+    return d
+
+
+FAKE = ctypes.c_void_p(0x1000)   # never dereferenced: every call below fails validation first
+
+
+def test_invalid_arguments_are_reported_not_launched():
+    lib = aprict_amd.lib()
+    cfg = cbfp.ConfigProject().contents
+    chunk = np.array((0.5, (0, 0, 0), (4, 4, 4), (-1, -1, -1)), dtype=_abi.vol_tex_chunk_spec)
+    vd, sd = _desc((4, 4, 4)), _desc((3, 5, 5))
+    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), None, 3, _abi.struct_ptr(chunk), vd, FAKE, sd, FAKE, None, None)
+    assert st == _abi.AP_ERR_INVALID and b"specs" in lib.ap_last_error()
+    # view count disagrees with the sinogram
+    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 2, _abi.struct_ptr(chunk), vd, FAKE, sd, FAKE, None, None)
+    assert st == _abi.AP_ERR_INVALID and b"views" in lib.ap_last_error()
+    # chunk larger than the volume
+    big = np.array((0.5, (1, 0, 0), (4, 4, 4), (-1, -1, -1)), dtype=_abi.vol_tex_chunk_spec)
+    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 3, _abi.struct_ptr(big), vd, FAKE, sd, FAKE, None, None)
+    assert st == _abi.AP_ERR_INVALID and b"chunk" in lib.ap_last_error()
+    # bfloat16 sinograms are not supported
+    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE,
+                              _desc((3, 5, 5), _abi.AP_BF16), FAKE, None, None)
+    assert st == _abi.AP_ERR_UNSUPPORTED
+    # GBC presets are refused
+    bp = cbfp.ConfigBackproject(backproj_weighting_preset=7).contents
+    st = lib.ap_backproject(_abi.struct_ptr(bp), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, None, None, sd, FAKE, None)
+    assert st == _abi.AP_ERR_UNSUPPORTED and b"GBC" in lib.ap_last_error()
+    # multiplier flag without a multiplier
+    bp = cbfp.ConfigBackproject(apply_summand_multiplier=True).contents
+    st = lib.ap_backproject(_abi.struct_ptr(bp), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, None, None, sd, FAKE, None)
+    assert st == _abi.AP_ERR_INVALID
+    # ramp filter: in == out
+    st = lib.ap_ramp_filter_rows(sd, FAKE, sd, FAKE, 1.0, 0, 0, None)
+    assert st == _abi.AP_ERR_INVALID
+    # supersample out of range
+    st = lib.ap_voxelise_forbild(FAKE, 1, _abi.struct_ptr(chunk), vd, FAKE, 9, None)
+    assert st == _abi.AP_ERR_INVALID
+    with pytest.raises(ValueError):
+        _abi.check(_abi.AP_ERR_INVALID, "x")
+    with pytest.raises(NotImplementedError):
+        _abi.check(_abi.AP_ERR_UNSUPPORTED, "x")
+
+
+def test_python_layer_refuses_cpu_tensors():
+    torch = pytest.importorskip("torch")
+    from aprict_amd.volume import Volume
+
+    v = Volume(np.zeros((4, 4, 4), np.float32), voxwid=1.0, device="cpu")
+    pg = cbfp.ProjGeom.new_circular_scan(num_source_points=2, R=10.0, L_minus_R=10.0,
+                                         pix_widhei_at_isocenter=(0.5, 0.5), num_pix_widhei=(3, 3))
+    ps = cbfp.ProjectionSet(pg, data=torch.zeros(2, 3, 3), device="cpu")
+    with pytest.raises(ValueError, match="GPU"):
+        cbfp.project(cbfp.ConfigProject(), v, ps)

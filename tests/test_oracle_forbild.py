@@ -1,0 +1,95 @@
+"""Pins the FP64 analytic FORBILD oracle (CPU only): closed-form chords of
+spheres, boxes, ellipsoids and cylinders, clip planes, last-wins and
+additive compositing (FORBILD_Shape.cu:474-540), and convergence of the
+voxelised Siddon projection to the analytic one."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from aprict_amd import _abi
+from aprict_amd import phantoms as ph
+from aprict_amd.cb import flatpanel as cbfp
+from aprict_amd.volume import forbild
+from helpers import chunk_for, rel_l2
+from oracle import oracle as orc
+
+
+def one_ray(src, dst):
+    pg = cbfp.ProjGeom(source_pos=np.array([src]), det_pos=np.array([dst]), det_uvec=np.array([[0.0, 1, 0]]),
+                       det_vvec=np.array([[0.0, 0, 1]]), nu=1, nv=1)
+    return pg.to_proj_spec_array()
+
+
+def integral(shapes, src, dst, additive=False):
+    cfg = np.array((1.0, additive, 1, 1), dtype=_abi.config_cb_project_FORBILD)
+    return orc.project_forbild(cfg, one_ray(src, dst), forbild.stack(shapes), np.zeros((1, 1, 1)))[0, 0, 0]
+
+
+def test_sphere_chord():
+    s = forbild.sphere((1.0, 2.0, -1.0), 3.0, 1.5)
+    for d in (0.0, 1.0, 2.875):   # float32-exact offsets (the specs are float32)
+        got = integral([s], (-50, 2.0 + d, -1.0), (50, 2.0 + d, -1.0))
+        assert got == pytest.approx(1.5 * 2 * np.sqrt(9 - d * d), rel=1e-9)
+    assert integral([s], (-50, 5.5, 0), (50, 5.5, 0)) == 0.0
+
+
+def test_box_and_segment_ends():
+    b = forbild.box((0, 0, 0), 2.0, 4.0, 6.0, 2.0)
+    assert integral([b], (-10, 0.3, 0.2), (10, 0.3, 0.2)) == pytest.approx(2 * 2.0, rel=1e-12)
+    # segment ends inside the box: only the covered part counts
+    assert integral([b], (-10, 0, 0), (0.25, 0, 0)) == pytest.approx(2 * 1.25, rel=1e-12)
+    d = np.array([1.0, 1.0, 0.0]) / np.sqrt(2)
+    assert integral([b], tuple(-10 * d), tuple(10 * d)) == pytest.approx(2.0 * 2 * np.sqrt(2), rel=1e-7)
+
+
+def test_ellipsoids_and_cylinders():
+    rot = ph.haar_rotation(np.random.default_rng(0))
+    ef = forbild.ellipsoid_free((0.5, 0, 0), (1.0, 2.0, 3.0), rot.T, 1.0)
+    # along the second principal axis the chord is twice that semi-axis
+    a = rot[:, 1]
+    c = np.array([0.5, 0, 0])
+    assert integral([ef], tuple(c - 20 * a), tuple(c + 20 * a)) == pytest.approx(4.0, rel=1e-9)
+    e10 = forbild.ellipsoid((0, 0, 0), 2.0, 2.0, 2.0, 1.0)
+    sp = forbild.sphere((0, 0, 0), 2.0, 1.0)
+    assert integral([e10], (-9, 0.7, 0.3), (9, 0.7, 0.3)) == pytest.approx(integral([sp], (-9, 0.7, 0.3), (9, 0.7, 0.3)), rel=1e-12)
+    cyl = forbild.cylinder((0, 0, 0), 1.5, 4.0, (0, 0, 1), 1.0)
+    assert integral([cyl], (-9, 0, 0.5), (9, 0, 0.5)) == pytest.approx(3.0, rel=1e-12)
+    assert integral([cyl], (0.2, 0, -9), (0.2, 0, 9)) == pytest.approx(4.0, rel=1e-12)
+    ec = forbild.ellipt_cyl((0, 0, 0), 1.0, 2.5, 3.0, (1, 0, 0), (0, 1, 0), (0, 0, 1), 1.0)
+    assert integral([ec], (0, -9, 0), (0, 9, 0)) == pytest.approx(5.0, rel=1e-12)
+
+
+def test_clip_planes():
+    half = forbild.sphere((0, 0, 0), 2.0, 1.0, clips=[((1.0, 0, 0), 0.0, 1)])   # keep x <= 0
+    assert integral([half], (-9, 0, 0), (9, 0, 0)) == pytest.approx(2.0, rel=1e-12)
+    other = forbild.sphere((0, 0, 0), 2.0, 1.0, clips=[((1.0, 0, 0), 0.5, 2)])  # keep x > 0.5
+    assert integral([other], (-9, 0, 0), (9, 0, 0)) == pytest.approx(1.5, rel=1e-12)
+
+
+def test_last_wins_and_additive():
+    big = forbild.sphere((0, 0, 0), 3.0, 1.0)
+    small = forbild.sphere((0, 0, 0), 1.0, 4.0)
+    ray = ((-9, 0, 0), (9, 0, 0))
+    assert integral([big, small], *ray) == pytest.approx(6 * 1.0 + 2 * (4.0 - 1.0), rel=1e-12)
+    assert integral([small, big], *ray) == pytest.approx(6.0, rel=1e-12)
+    assert integral([big, small], *ray, additive=True) == pytest.approx(6.0 + 8.0, rel=1e-12)
+
+
+def test_voxelised_projection_converges_to_analytic():
+    shapes = ph.random_primitives(6, 8.0, seed=13)
+    pg = ph.circular_geometry(6, 24, 24, 1.0, R=60.0, L=120.0)
+    specs = pg.to_proj_spec_array()
+    cfg_fb = np.array((1.0, False, 1, 1), dtype=_abi.config_cb_project_FORBILD)
+    exact = orc.project_forbild(cfg_fb, specs, shapes, np.zeros((6, 24, 24)))
+    errs = []
+    for n in (32, 64):
+        w = 22.0 / n
+        ch = chunk_for((n, n, n), w)
+        vol = orc.voxelise_forbild(shapes, ch, np.zeros((n, n, n)), ss=2)
+        got = orc.project_voxel(cbfp.ConfigProject().contents, specs, ch, vol, np.zeros((6, 24, 24)))
+        errs.append(rel_l2(got, exact))
+    assert np.abs(exact).max() > 1.0
+    # first-order convergence in the voxel width
+    assert errs[1] < 0.6 * errs[0] and errs[1] < 0.2
