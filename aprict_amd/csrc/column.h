@@ -1,0 +1,207 @@
+// column.h -- column-coherent Siddon traversal for detectors whose v axis is
+// parallel to z (circular and helical scans with upright panels).
+//
+// For such a view every ray through detector column ju has the same
+// projection onto the x-y plane: rays of one column differ only in their
+// slope along z.  Parametrising a ray by the horizontal distance h along that
+// common 2-D line, the x- and y-plane crossings are shared by the whole
+// column and only the z-plane crossings depend on the row kv.  The exact
+// Siddon integral of a ray is
+//     sum over voxels of (h_out - h_in) * sec * f,   sec = sqrt(1 + m^2),
+// m the ray's z slope per unit h.  The shared x-y crossings are merged once
+// per column into a segment list in LDS -- every crossing's rank in the merged
+// order is computed independently, so the list is built in parallel -- and
+// each lane walks the list adding its own z crossings.
+//
+// All crossing parameters are single FP32 fma expressions of FP32 constants
+// rounded once from FP64, shared by the forward projector and the adjoint, so
+// both see bit-identical segment boundaries and weights.
+#pragma once
+
+#include "siddon.h"
+
+// The shared 2-D line of one detector (sub-)column.
+struct Col2D {
+    float r[2], c[2], P0[2];   // h_a(P) = fma(P - P0_a, r_a, c_a), a = x, y
+    float u[2];                // unit direction in x-y
+    int sg[2];                 // direction signs (0: parallel to the planes)
+    int ifix[2];               // index along an axis with sg == 0
+    float hs, hd;              // source and pixel positions along h
+    double hc, lxy;            // closest-approach parameter, horizontal length
+    double sz, qz0, vz;        // source z, pixel z at kv = 0, z step per row
+};
+
+// Per-row z walk.
+struct LaneZ {
+    float r, c, P0;            // h_z(P) = fma(P - P0, r, c)
+    float u;                   // dz/dh as float (position estimates)
+    int sg, ifix;
+    float sec;                 // 3-D length per unit h
+};
+
+// 2-D set-up of column ju (FP64, rounded once).  lo/hi: the chunk box.
+// Returns false when the column's rays cannot meet the chunk in x-y.
+__device__ inline bool col_setup(const ViewRays& vr, const int lo[3], const int hi[3], float ju, Col2D& C) {
+    const double qx = vr.Q00[0] + (double)ju * vr.U[0];
+    const double qy = vr.Q00[1] + (double)ju * vr.U[1];
+    const double dx = qx - vr.S[0], dy = qy - vr.S[1];
+    const double l = sqrt(dx * dx + dy * dy);
+    if (!(l > 0.0)) return false;
+    const double u[2] = {dx / l, dy / l};
+    const double hc = -(vr.S[0] * u[0] + vr.S[1] * u[1]);
+    C.hc = hc;
+    C.lxy = l;
+    C.sz = vr.S[2];
+    C.qz0 = vr.Q00[2] + (double)ju * vr.U[2];
+    C.vz = vr.V[2];
+    C.hs = (float)(-hc);
+    C.hd = (float)(l - hc);
+    for (int a = 0; a < 2; a++) {
+        const double oab = vr.S[a] + hc * u[a] + 0.5 * (double)(lo[a] + hi[a]);
+        C.u[a] = (float)u[a];
+        if (fabs(u[a]) < 1e-20) {
+            C.sg[a] = 0;
+            C.r[a] = 0.f;
+            C.c[a] = AP_BIG;
+            C.P0[a] = 0.f;
+            if (!(oab > (double)lo[a] && oab <= (double)hi[a])) return false;
+            C.ifix[a] = (int)ceil(oab) - 1;
+            continue;
+        }
+        const double P0 = floor(oab);
+        C.P0[a] = (float)P0;
+        C.sg[a] = u[a] > 0.0 ? 1 : -1;
+        C.r[a] = (float)(1.0 / u[a]);
+        C.c[a] = (float)((P0 - oab) / u[a]);
+        C.ifix[a] = 0;
+    }
+    return true;
+}
+
+// Parameters [h0, h1) of the column line inside the x-y box [blo, bhi),
+// intersected with the source->pixel segment.
+__device__ __forceinline__ void col_box(const Col2D& C, const int blo[2], const int bhi[2], float& h0,
+                                        float& h1) {
+    h0 = C.hs;
+    h1 = C.hd;
+    for (int a = 0; a < 2; a++) {
+        if (C.sg[a] == 0) {
+            if (C.ifix[a] < blo[a] || C.ifix[a] >= bhi[a]) h1 = -AP_BIG;
+            continue;
+        }
+        const float ein = (float)(C.sg[a] > 0 ? blo[a] : bhi[a]) - C.P0[a];
+        const float eout = (float)(C.sg[a] > 0 ? bhi[a] : blo[a]) - C.P0[a];
+        h0 = fmaxf(h0, tcross(ein, C.r[a], C.c[a]));
+        h1 = fminf(h1, tcross(eout, C.r[a], C.c[a]));
+    }
+}
+
+// Index along one axis after every plane whose crossing parameter is <= h,
+// clamped to [blo, bhi).  (sg == 0: the fixed index.)
+__device__ __forceinline__ int axis_index_at(float r, float c, float P0, float u, int sg, int ifix, float h,
+                                             int blo, int bhi) {
+    if (sg == 0) return ifix;
+    int i = (int)(P0 + floorf((h - c) * u));
+    i = min(max(i, blo), bhi - 1);
+    if (sg > 0) {
+        if (i + 1 < bhi && tcross((float)(i + 1) - P0, r, c) <= h) i++;
+        if (i > blo && tcross((float)i - P0, r, c) > h) i--;
+    } else {
+        if (i > blo && tcross((float)i - P0, r, c) <= h) i--;
+        if (i + 1 < bhi && tcross((float)(i + 1) - P0, r, c) > h) i++;
+    }
+    return i;
+}
+
+// Crossings of one axis strictly inside (h0, h1) along the column line,
+// starting from the voxel i0 occupied at h0: the m-th is at
+// tcross(first + sg*m - P0).
+struct AxisSeq {
+    float r, c, P0;
+    int first, sg, n;
+};
+
+__device__ __forceinline__ float seq_at(const AxisSeq& q, int m) {
+    return tcross((float)(q.first + q.sg * m) - q.P0, q.r, q.c);
+}
+
+__device__ inline AxisSeq axis_seq(const Col2D& C, int a, int i0, float h1, int blo, int bhi) {
+    AxisSeq q;
+    q.r = C.r[a];
+    q.c = C.c[a];
+    q.P0 = C.P0[a];
+    q.sg = C.sg[a];
+    if (C.sg[a] == 0) {
+        q.first = 0;
+        q.n = 0;
+        return q;
+    }
+    q.first = C.sg[a] > 0 ? i0 + 1 : i0;
+    // planes strictly before the box's exit plane
+    const int avail = C.sg[a] > 0 ? bhi - q.first : q.first - blo;
+    if (avail <= 0) {
+        q.n = 0;
+        return q;
+    }
+    const float x0 = seq_at(q, 0);
+    int n = (int)floorf((h1 - x0) * fabsf(C.u[a])) + 1;
+    n = min(max(n, 0), avail);
+    while (n < avail && seq_at(q, n) < h1) n++;
+    while (n > 0 && !(seq_at(q, n - 1) < h1)) n--;
+    q.n = n;
+    return q;
+}
+
+// Number of crossings of q strictly below h (strict) or at most h (!strict).
+__device__ __forceinline__ int seq_count(const AxisSeq& q, float h, bool strict, float absu) {
+    if (q.n == 0) return 0;
+    int k = (int)floorf((h - seq_at(q, 0)) * absu) + 1;
+    k = min(max(k, 0), q.n);
+    if (strict) {
+        while (k < q.n && seq_at(q, k) < h) k++;
+        while (k > 0 && !(seq_at(q, k - 1) < h)) k--;
+    } else {
+        while (k < q.n && seq_at(q, k) <= h) k++;
+        while (k > 0 && !(seq_at(q, k - 1) <= h)) k--;
+    }
+    return k;
+}
+
+// z walk of row kv of column C (FP64 set-up).  Returns false for a ray that
+// cannot meet the z range [zlo, zhi) (horizontal rays outside it).
+__device__ inline bool lane_setup(const Col2D& C, float kv, int zlo, int zhi, LaneZ& Z) {
+    const double qz = C.qz0 + (double)kv * C.vz;
+    const double m = (qz - C.sz) / C.lxy;
+    const double zab = C.sz + C.hc * m + 0.5 * (double)(zlo + zhi);
+    Z.sec = (float)sqrt(1.0 + m * m);
+    Z.u = (float)m;
+    if (fabs(m) < 1e-20) {
+        Z.sg = 0;
+        Z.r = 0.f;
+        Z.c = AP_BIG;
+        Z.P0 = 0.f;
+        if (!(zab > (double)zlo && zab <= (double)zhi)) return false;
+        Z.ifix = (int)ceil(zab) - 1;
+        return true;
+    }
+    const double P0 = floor(zab);
+    Z.P0 = (float)P0;
+    Z.sg = m > 0.0 ? 1 : -1;
+    Z.r = (float)(1.0 / m);
+    Z.c = (float)((P0 - zab) / m);
+    Z.ifix = 0;
+    return true;
+}
+
+// h range of the row inside the z slab [blo, bhi)
+__device__ __forceinline__ void lane_box(const LaneZ& Z, int blo, int bhi, float& h0, float& h1) {
+    if (Z.sg == 0) {
+        if (Z.ifix < blo || Z.ifix >= bhi) { h0 = AP_BIG; h1 = -AP_BIG; }
+        else { h0 = -AP_BIG; h1 = AP_BIG; }
+        return;
+    }
+    const float ein = (float)(Z.sg > 0 ? blo : bhi) - Z.P0;
+    const float eout = (float)(Z.sg > 0 ? bhi : blo) - Z.P0;
+    h0 = tcross(ein, Z.r, Z.c);
+    h1 = tcross(eout, Z.r, Z.c);
+}

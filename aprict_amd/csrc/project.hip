@@ -1,46 +1,103 @@
 // project.hip -- ray-driven Siddon forward projector (ap_project_voxel) and
-// its matched adjoint (ap_backproject_matched).
+// its exact transpose (ap_backproject_matched).
 //
-// Thread mapping: one thread per detector pixel; a wavefront holds 64
-// consecutive pixels along v (the fastest sinogram axis, and z, the fastest
-// volume axis, for the circular geometries this path serves), so the
-// per-step volume gathers of a wave fall in a few cache lines and the
-// sinogram stores are fully coalesced.
+// Forward: one thread per detector pixel; a wavefront holds 64 consecutive
+// pixels along v (the fastest sinogram axis, and z, the fastest volume axis,
+// for the circular geometries this path serves), so the per-step gathers of a
+// wave fall in a few cache lines and the sinogram stores are coalesced.  Each
+// thread walks kBatch voxels before issuing their kBatch loads, so several
+// gathers per wave are in flight.  Tiles are ordered view-group / u-tile /
+// view / v-tile and split into one contiguous range per XCD, so the blocks
+// resident at any time sweep the same slab of the volume for several
+// neighbouring views out of the same L2.
+//
+// Adjoint: one workgroup per volume brick accumulates A^T y in LDS over all
+// views.  Per view it visits the detector pixels whose rays can meet the
+// brick (bounding box of the projected brick corners), clips each ray to the
+// brick with the same crossing parameters as the forward walk and scatters
+// weight*y with LDS float atomics; the brick is written to HBM once.  No
+// global atomics, one write per voxel.
 #include "siddon.h"
 
 namespace {
 
 constexpr int kWaveV = 64;   // pixels along v per wave
 constexpr int kRowsU = 4;    // waves (rows along u) per block
+constexpr int kBatch = 4;    // voxels walked per batch of gathers
+constexpr int kViewGroup = 8;
+constexpr int kXcd = 8;      // MI355X: 8 XCDs, workgroups dispatched round-robin
 
 struct SinoCfg {
     uint32_t V, nu, nv;
     int64_t s0, s1, s2;      // element strides
 };
 
+struct TileMap {
+    uint32_t nk, nj;         // v-tiles, u-tiles per view
+    uint32_t total;          // padded tile count (multiple of view groups)
+    uint32_t per_xcd;        // tiles per XCD
+};
+
+__device__ __forceinline__ bool map_tile(const TileMap& tm, uint32_t V, uint32_t& p, uint32_t& jt,
+                                         uint32_t& kt) {
+    const uint32_t L = blockIdx.x;
+    const uint32_t logical = (L % kXcd) * tm.per_xcd + L / kXcd;
+    if (logical >= tm.total) return false;
+    kt = logical % tm.nk;
+    uint32_t r = logical / tm.nk;
+    const uint32_t vi = r % kViewGroup;
+    r /= kViewGroup;
+    jt = r % tm.nj;
+    const uint32_t vg = r / tm.nj;
+    p = vg * kViewGroup + vi;
+    return p < V;
+}
+
+__device__ __forceinline__ float sub_offset(uint32_t s, uint32_t ss) {
+    return -0.5f + ((float)s + 0.5f) / (float)ss;
+}
+
 template <typename Tv, typename Tp, typename IdxT>
 __global__ __launch_bounds__(kWaveV* kRowsU) void k_project_voxel(
-    RayCfg rc, SinoCfg sc, float mba, uint32_t ssu, uint32_t ssv,
-    const ap_proj_spec* __restrict__ specs, const Tv* __restrict__ vol,
-    IdxT vsx, IdxT vsy, IdxT vsz, Tp* __restrict__ sino, const Tp* __restrict__ b) {
-    const uint32_t k = blockIdx.x * kWaveV + threadIdx.x;
-    const uint32_t j = blockIdx.y * kRowsU + threadIdx.y;
-    const uint32_t p = blockIdx.z;
+    RayCfg rc, SinoCfg sc, TileMap tm, float mba, uint32_t ssu, uint32_t ssv,
+    const ap_proj_spec* __restrict__ specs, const Tv* __restrict__ vol, IdxT vsx, IdxT vsy, IdxT vsz,
+    Tp* __restrict__ sino, const Tp* __restrict__ b, int skip_upright) {
+    uint32_t p, jt, kt;
+    if (!map_tile(tm, sc.V, p, jt, kt)) return;
+    if (skip_upright && view_is_upright(specs, p)) return;
+    const uint32_t k = kt * kWaveV + threadIdx.x;
+    const uint32_t j = jt * kRowsU + threadIdx.y;
     if (k >= sc.nv || j >= sc.nu) return;
-    const ViewGeom g = load_view(specs, p);
-    const d3 S = to_vox(rc, g.S);
+    const ViewRays vr = make_view_rays(rc, specs, p);
     float tot = 0.f;
     for (uint32_t su = 0; su < ssu; su++) {
-        float acc = 0.f;
+        float accu = 0.f;
         for (uint32_t sv = 0; sv < ssv; sv++) {
-            const double ju = (double)j - 0.5 + ((double)su + 0.5) / (double)ssu;
-            const double kv = (double)k - 0.5 + ((double)sv + 0.5) / (double)ssv;
-            const d3 Q = to_vox(rc, pixel_point(g, ju, kv));
+            const float ju = (float)j + sub_offset(su, ssu);
+            const float kv = (float)k + sub_offset(sv, ssv);
             Ray R;
-            if (ray_setup(rc, S, Q, R))
-                acc += ray_integral<Tv, IdxT>(vol, R, vsx, vsy, vsz, rc.maxit);
+            if (!ray_setup(vr, rc.lo, rc.hi, ju, kv, R)) continue;
+            ray_start(R, R.t_enter, rc.lo, rc.hi);
+            Walker<IdxT> W;
+            W.init(R, R.t_enter, R.t_exit, vsx, vsy, vsz,
+                   (IdxT)R.idx[0] * vsx + (IdxT)R.idx[1] * vsy + (IdxT)R.idx[2] * vsz);
+            float acc = 0.f;
+            for (int it = 0; it < rc.maxit; it += kBatch) {
+                float w[kBatch];
+                IdxT o[kBatch];
+                bool more = true;
+#pragma unroll
+                for (int q = 0; q < kBatch; q++) more = W.step(w[q], o[q]);
+                float v[kBatch];
+#pragma unroll
+                for (int q = 0; q < kBatch; q++) v[q] = to_f32(vol[o[q]]);
+#pragma unroll
+                for (int q = 0; q < kBatch; q++) acc = __builtin_fmaf(w[q], v[q], acc);
+                if (!more) break;
+            }
+            accu += acc;
         }
-        tot += ssv > 1 ? acc / (float)ssv : acc;
+        tot += ssv > 1 ? accu / (float)ssv : accu;
     }
     if (ssu > 1) tot /= (float)ssu;
     tot *= (float)(1.0 / rc.inv_vw);   // voxel-length units -> physical
@@ -49,61 +106,163 @@ __global__ __launch_bounds__(kWaveV* kRowsU) void k_project_voxel(
     sino[o] = from_f32<Tp>(__builtin_fmaf(mba, tot, base));
 }
 
-// Adjoint by scatter: every sample adds weight*y into the volume with a
-// hardware float atomic.  vol must be F32.
-template <typename Tp, typename Tm, typename IdxT>
-__global__ __launch_bounds__(kWaveV* kRowsU) void k_backproject_matched_atomic(
+// ---------------------------------------------------------------- adjoint
+constexpr int kBX = 32, kBY = 32, kBZ = 8;     // brick (voxels), z fastest in LDS
+constexpr int kAdjThreads = 256;
+constexpr int kViewBatchAdj = 32;
+
+struct BrickView {
+    ViewRays vr;
+    int j0, j1, k0, k1;   // detector footprint (inclusive), empty when j1 < j0
+};
+
+// Pixel coordinates of the central projection of X (voxel units rel. centre):
+// X = S + lam (ju U + kv V + Q00 - S) solved with the dual basis of (U, V).
+__device__ __forceinline__ bool project_point(const ViewRays& vd, const float X[3], float& ju, float& kv) {
+    struct { float S[3], Q00[3], U[3], V[3]; } vr;
+    for (int a = 0; a < 3; a++) {
+        vr.S[a] = (float)vd.S[a];
+        vr.Q00[a] = (float)vd.Q00[a];
+        vr.U[a] = (float)vd.U[a];
+        vr.V[a] = (float)vd.V[a];
+    }
+    const float n[3] = {vr.U[1] * vr.V[2] - vr.U[2] * vr.V[1], vr.U[2] * vr.V[0] - vr.U[0] * vr.V[2],
+                        vr.U[0] * vr.V[1] - vr.U[1] * vr.V[0]};
+    float xs[3], qs[3];
+    for (int a = 0; a < 3; a++) {
+        xs[a] = X[a] - vr.S[a];
+        qs[a] = vr.Q00[a] - vr.S[a];
+    }
+    const float den = xs[0] * n[0] + xs[1] * n[1] + xs[2] * n[2];
+    const float num = qs[0] * n[0] + qs[1] * n[1] + qs[2] * n[2];
+    if (!(den * num > 0.f)) return false;    // point not in front of the source
+    const float lam = num / den;
+    float P[3];
+    for (int a = 0; a < 3; a++) P[a] = vr.S[a] + lam * xs[a] - vr.Q00[a];
+    const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    // ju = ((P x V) . n) / |n|^2,  kv = ((U x P) . n) / |n|^2
+    const float pv[3] = {P[1] * vr.V[2] - P[2] * vr.V[1], P[2] * vr.V[0] - P[0] * vr.V[2],
+                         P[0] * vr.V[1] - P[1] * vr.V[0]};
+    const float up[3] = {vr.U[1] * P[2] - vr.U[2] * P[1], vr.U[2] * P[0] - vr.U[0] * P[2],
+                         vr.U[0] * P[1] - vr.U[1] * P[0]};
+    ju = (pv[0] * n[0] + pv[1] * n[1] + pv[2] * n[2]) / nn;
+    kv = (up[0] * n[0] + up[1] * n[1] + up[2] * n[2]) / nn;
+    return true;
+}
+
+template <typename Tv, typename Tp, typename Tm, typename IdxT>
+__global__ __launch_bounds__(kAdjThreads) void k_backproject_matched(
     RayCfg rc, SinoCfg sc, float scale, uint32_t ssu, uint32_t ssv,
-    const ap_proj_spec* __restrict__ specs, float* __restrict__ vol,
-    IdxT vsx, IdxT vsy, IdxT vsz, const Tm* __restrict__ mult,
-    IdxT msx, IdxT msy, IdxT msz, const Tp* __restrict__ sino) {
-    const uint32_t k = blockIdx.x * kWaveV + threadIdx.x;
-    const uint32_t j = blockIdx.y * kRowsU + threadIdx.y;
-    const uint32_t p = blockIdx.z;
-    if (k >= sc.nv || j >= sc.nu) return;
-    const int64_t o = (int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2;
-    const float y = to_f32(sino[o]);
-    if (y == 0.f) return;
-    const float wy = y * scale;
-    const ViewGeom g = load_view(specs, p);
-    const d3 S = to_vox(rc, g.S);
-    for (uint32_t su = 0; su < ssu; su++)
-        for (uint32_t sv = 0; sv < ssv; sv++) {
-            const double ju = (double)j - 0.5 + ((double)su + 0.5) / (double)ssu;
-            const double kv = (double)k - 0.5 + ((double)sv + 0.5) / (double)ssv;
-            const d3 Q = to_vox(rc, pixel_point(g, ju, kv));
-            Ray R;
-            if (!ray_setup(rc, S, Q, R)) continue;
-            // walk the ray in both the volume and the multiplier index spaces
-            IdxT offm = (IdxT)R.idx[0] * msx + (IdxT)R.idx[1] * msy + (IdxT)R.idx[2] * msz;
-            const IdxT dmx = R.sg[0] >= 0 ? msx : -msx;
-            const IdxT dmy = R.sg[1] >= 0 ? msy : -msy;
-            const IdxT dmz = R.sg[2] >= 0 ? msz : -msz;
-            IdxT offv = (IdxT)R.idx[0] * vsx + (IdxT)R.idx[1] * vsy + (IdxT)R.idx[2] * vsz;
-            const IdxT dvx = R.sg[0] >= 0 ? vsx : -vsx;
-            const IdxT dvy = R.sg[1] >= 0 ? vsy : -vsy;
-            const IdxT dvz = R.sg[2] >= 0 ? vsz : -vsz;
-            const float gx = (float)R.sg[0], gy = (float)R.sg[1], gz = (float)R.sg[2];
-            float tx = R.tn[0], ty = R.tn[1], tz = R.tn[2];
-            float px = R.pn[0], py = R.pn[1], pz = R.pn[2];
-            float tprev = R.t_enter;
-            for (int it = 0; it < rc.maxit; ++it) {
-                const float tmn = fminf(fminf(tx, ty), tz);
-                const float w = fminf(tmn, R.t_exit) - tprev;
-                const float m = mult ? to_f32(mult[offm]) : 1.f;
-                atomicAdd(vol + offv, w * wy * m);
-                if (tmn >= R.t_exit) break;
-                tprev = tmn;
-                if (tx == tmn) {
-                    px += gx; tx = tcross(px, R.r[0], R.c[0]); offv += dvx; offm += dmx;
-                } else if (ty == tmn) {
-                    py += gy; ty = tcross(py, R.r[1], R.c[1]); offv += dvy; offm += dmy;
-                } else {
-                    pz += gz; tz = tcross(pz, R.r[2], R.c[2]); offv += dvz; offm += dmz;
-                }
+    const ap_proj_spec* __restrict__ specs, Tv* __restrict__ vol, IdxT vsx, IdxT vsy, IdxT vsz,
+    const Tm* __restrict__ mult, IdxT msx, IdxT msy, IdxT msz, const Tp* __restrict__ sino,
+    uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t per_xcd, int skip_upright) {
+    __shared__ float acc[kBX * kBY * kBZ];
+    __shared__ BrickView bv[kViewBatchAdj];
+    const uint32_t L = blockIdx.x;
+    const uint32_t logical = (L % kXcd) * per_xcd + L / kXcd;
+    const uint32_t nbr = nbx * nby * nbz;
+    if (logical >= nbr) return;
+    // bricks ordered z fastest so an XCD's bricks form compact columns
+    const uint32_t bk = logical % nbz, bj = (logical / nbz) % nby, bi = logical / (nbz * nby);
+    int blo[3] = {rc.lo[0] + (int)bi * kBX, rc.lo[1] + (int)bj * kBY, rc.lo[2] + (int)bk * kBZ};
+    int bhi[3] = {min(blo[0] + kBX, rc.hi[0]), min(blo[1] + kBY, rc.hi[1]), min(blo[2] + kBZ, rc.hi[2])};
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kBX * kBY * kBZ; e += kAdjThreads) acc[e] = 0.f;
+    const int maxit = kBX + kBY + kBZ + 3;
+    const float half_u = 0.5f, half_v = 0.5f;
+    for (uint32_t p0 = 0; p0 < sc.V; p0 += kViewBatchAdj) {
+        const uint32_t nb = min((uint32_t)kViewBatchAdj, sc.V - p0);
+        __syncthreads();
+        if (tid < (int)nb && skip_upright && view_is_upright(specs, p0 + tid)) {
+            bv[tid].j0 = 0; bv[tid].j1 = -1; bv[tid].k0 = 0; bv[tid].k1 = -1;
+        } else if (tid < (int)nb) {
+            BrickView& B = bv[tid];
+            B.vr = make_view_rays(rc, specs, p0 + tid);
+            // footprint: bounding box of the projected brick corners, widened
+            // by half a pixel for sub-pixel rays
+            float jmin = AP_BIG, jmax = -AP_BIG, kmin = AP_BIG, kmax = -AP_BIG;
+            bool ok = true;
+            for (int q = 0; q < 8; q++) {
+                const float X[3] = {(float)((q & 4) ? bhi[0] : blo[0]) - 0.5f * (float)(rc.lo[0] + rc.hi[0]),
+                                    (float)((q & 2) ? bhi[1] : blo[1]) - 0.5f * (float)(rc.lo[1] + rc.hi[1]),
+                                    (float)((q & 1) ? bhi[2] : blo[2]) - 0.5f * (float)(rc.lo[2] + rc.hi[2])};
+                float ju, kv;
+                if (!project_point(B.vr, X, ju, kv)) { ok = false; break; }
+                jmin = fminf(jmin, ju); jmax = fmaxf(jmax, ju);
+                kmin = fminf(kmin, kv); kmax = fmaxf(kmax, kv);
+            }
+            if (ok) {
+                // 1e-3 pixel of slack absorbs the FP32 rounding of the projection
+                B.j0 = max(0, (int)floorf(jmin - half_u - 1e-3f));
+                B.j1 = min((int)sc.nu - 1, (int)ceilf(jmax + half_u + 1e-3f));
+                B.k0 = max(0, (int)floorf(kmin - half_v - 1e-3f));
+                B.k1 = min((int)sc.nv - 1, (int)ceilf(kmax + half_v + 1e-3f));
+            } else {   // brick not entirely in front of the source: every pixel
+                B.j0 = 0; B.j1 = (int)sc.nu - 1; B.k0 = 0; B.k1 = (int)sc.nv - 1;
             }
         }
+        __syncthreads();
+        for (uint32_t q = 0; q < nb; q++) {
+            const BrickView& B = bv[q];
+            if (B.j1 < B.j0 || B.k1 < B.k0) continue;
+            const int nk = B.k1 - B.k0 + 1;
+            const int npx = (B.j1 - B.j0 + 1) * nk;
+            const uint32_t p = p0 + q;
+            for (int pi = tid; pi < npx; pi += kAdjThreads) {
+                const int j = B.j0 + pi / nk, k = B.k0 + pi % nk;
+                const float y = to_f32(sino[(int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2]);
+                if (y == 0.f) continue;
+                const float ys = y * scale;
+                for (uint32_t su = 0; su < ssu; su++)
+                    for (uint32_t sv = 0; sv < ssv; sv++) {
+                        Ray R;
+                        if (!ray_setup(B.vr, rc.lo, rc.hi, (float)j + sub_offset(su, ssu),
+                                       (float)k + sub_offset(sv, ssv), R))
+                            continue;
+                        float tb0, tb1;
+                        ray_box(R, blo, bhi, tb0, tb1);
+                        if (!(tb0 < tb1)) continue;
+                        ray_start(R, tb0, blo, bhi);
+                        Walker<int> W;
+                        W.init(R, tb0, tb1, kBY * kBZ, kBZ, 1,
+                               ((R.idx[0] - blo[0]) * kBY + (R.idx[1] - blo[1])) * kBZ + (R.idx[2] - blo[2]));
+                        for (int it = 0; it < maxit; it++) {
+                            float w;
+                            int o;
+                            const bool more = W.step(w, o);
+                            atomicAdd(&acc[o], w * ys);
+                            if (!more) break;
+                        }
+                    }
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < kBX * kBY * kBZ; e += kAdjThreads) {
+        const int li = e / (kBY * kBZ), lj = (e / kBZ) % kBY, lk = e % kBZ;
+        const int gi = blo[0] + li, gj = blo[1] + lj, gk = blo[2] + lk;
+        if (gi >= bhi[0] || gj >= bhi[1] || gk >= bhi[2]) continue;
+        const float a = acc[e];
+        if (a == 0.f) continue;
+        const IdxT ov = (IdxT)gi * vsx + (IdxT)gj * vsy + (IdxT)gk * vsz;
+        const float m = mult ? to_f32(mult[(IdxT)gi * msx + (IdxT)gj * msy + (IdxT)gk * msz]) : 1.f;
+        vol[ov] = from_f32<Tv>(__builtin_fmaf(m, a, to_f32(vol[ov])));
+    }
 }
+
+}  // namespace
+
+int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32_t nv, int64_t s0, int64_t s1,
+                               int64_t s2, float mba, uint32_t ssu, uint32_t ssv, const ap_proj_spec* specs,
+                               const ap_array3* vol_desc, const void* vol, int32_t sino_dtype, void* sino,
+                               const void* b, bool idx64, hipStream_t s);
+int32_t launch_backproject_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32_t nv, int64_t s0,
+                                   int64_t s1, int64_t s2, float scale, uint32_t ssu, uint32_t ssv,
+                                   const ap_proj_spec* specs, const ap_array3* vol_desc, void* vol,
+                                   int32_t mdt, const int64_t* ms, const void* mult, int32_t sino_dtype,
+                                   const void* sino, bool idx64, hipStream_t s);
+
+namespace {
 
 int32_t check_sino(const ap_array3* sd, const void* sino, uint32_t num_projs, SinoCfg& sc,
                    bool allow_f64) {
@@ -135,6 +294,7 @@ int32_t make_ray_cfg(const ap_config_cb_project* cfg, const ap_vol_tex_chunk_spe
         AP_REQUIRE((uint64_t)off[a] + shp[a] <= vol_desc->shape[a],
                    "chunk exceeds the volume along axis %d", a);
         AP_REQUIRE(shp[a] >= 1, "empty chunk along axis %d", a);
+        AP_REQUIRE((uint64_t)off[a] + shp[a] < (1u << 24), "volume extent above 2^24 voxels");
         rc.lo[a] = (int)off[a];
         rc.hi[a] = (int)(off[a] + shp[a]);
         ext += (int)shp[a];
@@ -161,13 +321,26 @@ extern "C" int32_t ap_project_voxel(const ap_config_cb_project* cfg, const ap_pr
     RayCfg rc;
     if ((st = make_ray_cfg(cfg, chunk, vol_desc, rc))) return st;
     if (num_projs == 0 || sc.nu == 0 || sc.nv == 0) return AP_OK;
-    AP_REQUIRE(num_projs <= 65535u, "at most 65535 views per call");
     const int64_t maxoff = array3_max_offset(vol_desc);
     const bool idx64 = maxoff >= (int64_t)INT32_MAX;
+    TileMap tm;
+    tm.nk = (sc.nv + kWaveV - 1) / kWaveV;
+    tm.nj = (sc.nu + kRowsU - 1) / kRowsU;
+    const uint64_t groups = (num_projs + kViewGroup - 1) / kViewGroup;
+    const uint64_t total = groups * kViewGroup * tm.nj * tm.nk;
+    AP_REQUIRE(total < (1ull << 31), "too many tiles in one call");
+    tm.total = (uint32_t)total;
+    tm.per_xcd = (tm.total + kXcd - 1) / kXcd;
     dim3 block(kWaveV, kRowsU, 1);
-    dim3 grid((sc.nv + kWaveV - 1) / kWaveV, (sc.nu + kRowsU - 1) / kRowsU, num_projs);
+    dim3 grid(tm.per_xcd * kXcd, 1, 1);
     hipStream_t s = (hipStream_t)stream;
     const float mba = cfg->multiply_before_adding;
+    // upright views go to the column-coherent kernel; the general kernel
+    // then only takes the remaining views
+    st = launch_project_columns(rc, sc.V, sc.nu, sc.nv, sc.s0, sc.s1, sc.s2, mba, cfg->ssu, cfg->ssv, specs,
+                                vol_desc, vol, sino_desc->dtype, sino, b, idx64, s);
+    if (st != AP_OK && st != AP_ERR_UNSUPPORTED) return st;
+    const int skip = st == AP_OK ? 1 : 0;
     return dispatch_dtype(vol_desc->dtype, [&](auto tv) -> int32_t {
         using Tv = decltype(tv);
         if constexpr (std::is_same<Tv, double>::value) {
@@ -179,15 +352,15 @@ extern "C" int32_t ap_project_voxel(const ap_config_cb_project* cfg, const ap_pr
                     return ap_fail(AP_ERR_UNSUPPORTED, "sinogram dtype must be F32 or F16");
                 } else {
                     if (idx64) {
-                        hipLaunchKernelGGL((k_project_voxel<Tv, Tp, int64_t>), grid, block, 0, s, rc, sc,
+                        hipLaunchKernelGGL((k_project_voxel<Tv, Tp, int64_t>), grid, block, 0, s, rc, sc, tm,
                                            mba, cfg->ssu, cfg->ssv, specs, (const Tv*)vol,
                                            vol_desc->strides[0], vol_desc->strides[1],
-                                           vol_desc->strides[2], (Tp*)sino, (const Tp*)b);
+                                           vol_desc->strides[2], (Tp*)sino, (const Tp*)b, skip);
                     } else {
-                        hipLaunchKernelGGL((k_project_voxel<Tv, Tp, int32_t>), grid, block, 0, s, rc, sc,
+                        hipLaunchKernelGGL((k_project_voxel<Tv, Tp, int32_t>), grid, block, 0, s, rc, sc, tm,
                                            mba, cfg->ssu, cfg->ssv, specs, (const Tv*)vol,
                                            (int32_t)vol_desc->strides[0], (int32_t)vol_desc->strides[1],
-                                           (int32_t)vol_desc->strides[2], (Tp*)sino, (const Tp*)b);
+                                           (int32_t)vol_desc->strides[2], (Tp*)sino, (const Tp*)b, skip);
                     }
                     return ap_check_launch("project_voxel");
                 }
@@ -204,7 +377,7 @@ extern "C" int32_t ap_backproject_matched(const ap_config_cb_project* cfg,
                                           const ap_array3* sino_desc, const void* sino,
                                           float alpha, void* stream) {
     AP_REQUIRE(specs, "specs is NULL");
-    int32_t st = ap_check_array3(vol_desc, vol, "volume", false, false, false);
+    int32_t st = ap_check_array3(vol_desc, vol, "volume", true, true, false);
     if (st) return st;
     if (multiplier) {
         if ((st = ap_check_array3(mult_desc, multiplier, "multiplier", true, true, false))) return st;
@@ -216,44 +389,52 @@ extern "C" int32_t ap_backproject_matched(const ap_config_cb_project* cfg,
     RayCfg rc;
     if ((st = make_ray_cfg(cfg, chunk, vol_desc, rc))) return st;
     if (num_projs == 0 || sc.nu == 0 || sc.nv == 0) return AP_OK;
-    AP_REQUIRE(num_projs <= 65535u, "at most 65535 views per call");
     const int64_t maxoff = array3_max_offset(vol_desc);
     const int64_t maxoffm = multiplier ? array3_max_offset(mult_desc) : 0;
     const bool idx64 = maxoff >= (int64_t)INT32_MAX || maxoffm >= (int64_t)INT32_MAX;
     const float scale = (float)((double)alpha * cfg->multiply_before_adding /
                                 ((double)cfg->ssu * cfg->ssv) * chunk->voxwid);
-    dim3 block(kWaveV, kRowsU, 1);
-    dim3 grid((sc.nv + kWaveV - 1) / kWaveV, (sc.nu + kRowsU - 1) / kRowsU, num_projs);
+    const uint32_t nbx = (chunk->shape.x + kBX - 1) / kBX;
+    const uint32_t nby = (chunk->shape.y + kBY - 1) / kBY;
+    const uint32_t nbz = (chunk->shape.z + kBZ - 1) / kBZ;
+    const uint64_t nbr = (uint64_t)nbx * nby * nbz;
+    AP_REQUIRE(nbr < (1ull << 31), "too many bricks");
+    const uint32_t per_xcd = (uint32_t)((nbr + kXcd - 1) / kXcd);
+    dim3 grid(per_xcd * kXcd, 1, 1);
     hipStream_t s = (hipStream_t)stream;
     const int32_t mdt = multiplier ? mult_desc->dtype : AP_F32;
-    return dispatch_dtype(sino_desc->dtype, [&](auto tp) -> int32_t {
-        using Tp = decltype(tp);
-        if constexpr (std::is_same<Tp, double>::value || std::is_same<Tp, bf16_t>::value) {
-            return ap_fail(AP_ERR_UNSUPPORTED, "sinogram dtype must be F32 or F16");
-        } else {
+    const int64_t* ms = multiplier ? mult_desc->strides : vol_desc->strides;
+    st = launch_backproject_columns(rc, sc.V, sc.nu, sc.nv, sc.s0, sc.s1, sc.s2, scale, cfg->ssu, cfg->ssv,
+                                    specs, vol_desc, vol, mdt, ms, multiplier, sino_desc->dtype, sino, idx64, s);
+    if (st != AP_OK && st != AP_ERR_UNSUPPORTED) return st;
+    const int skip = st == AP_OK ? 1 : 0;
+    return dispatch_dtype(vol_desc->dtype, [&](auto tv) -> int32_t {
+        using Tv = decltype(tv);
+        return dispatch_dtype(sino_desc->dtype, [&](auto tp) -> int32_t {
+            using Tp = decltype(tp);
             return dispatch_dtype(mdt, [&](auto tm) -> int32_t {
                 using Tm = decltype(tm);
-                if constexpr (std::is_same<Tm, double>::value) {
-                    return ap_fail(AP_ERR_UNSUPPORTED, "float64 multiplier");
+                if constexpr (std::is_same<Tv, double>::value || std::is_same<Tp, double>::value ||
+                              std::is_same<Tp, bf16_t>::value || std::is_same<Tm, double>::value) {
+                    return ap_fail(AP_ERR_UNSUPPORTED, "unsupported dtype combination");
                 } else {
-                    const int64_t* ms = multiplier ? mult_desc->strides : vol_desc->strides;
                     if (idx64) {
-                        hipLaunchKernelGGL((k_backproject_matched_atomic<Tp, Tm, int64_t>), grid, block, 0, s,
-                                           rc, sc, scale, cfg->ssu, cfg->ssv, specs, (float*)vol,
-                                           vol_desc->strides[0], vol_desc->strides[1],
-                                           vol_desc->strides[2], (const Tm*)multiplier, ms[0], ms[1],
-                                           ms[2], (const Tp*)sino);
+                        hipLaunchKernelGGL((k_backproject_matched<Tv, Tp, Tm, int64_t>), grid, dim3(kAdjThreads),
+                                           0, s, rc, sc, scale, cfg->ssu, cfg->ssv, specs, (Tv*)vol,
+                                           vol_desc->strides[0], vol_desc->strides[1], vol_desc->strides[2],
+                                           (const Tm*)multiplier, ms[0], ms[1], ms[2], (const Tp*)sino, nbx,
+                                           nby, nbz, per_xcd, skip);
                     } else {
-                        hipLaunchKernelGGL((k_backproject_matched_atomic<Tp, Tm, int32_t>), grid, block, 0, s,
-                                           rc, sc, scale, cfg->ssu, cfg->ssv, specs, (float*)vol,
+                        hipLaunchKernelGGL((k_backproject_matched<Tv, Tp, Tm, int32_t>), grid, dim3(kAdjThreads),
+                                           0, s, rc, sc, scale, cfg->ssu, cfg->ssv, specs, (Tv*)vol,
                                            (int32_t)vol_desc->strides[0], (int32_t)vol_desc->strides[1],
                                            (int32_t)vol_desc->strides[2], (const Tm*)multiplier,
-                                           (int32_t)ms[0], (int32_t)ms[1], (int32_t)ms[2],
-                                           (const Tp*)sino);
+                                           (int32_t)ms[0], (int32_t)ms[1], (int32_t)ms[2], (const Tp*)sino,
+                                           nbx, nby, nbz, per_xcd, skip);
                     }
                     return ap_check_launch("backproject_matched");
                 }
             });
-        }
+        });
     });
 }
