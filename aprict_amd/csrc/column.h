@@ -20,6 +20,13 @@
 
 #include "siddon.h"
 
+// Per-column constants of the row z walks (FP64).
+struct ColZ {
+    double hc;                 // closest-approach parameter of the 2-D line
+    double il;                 // 1 / horizontal source-pixel distance
+    double sz, qz0, vz;        // source z, pixel z at kv = 0, z step per row
+};
+
 // The shared 2-D line of one detector (sub-)column.
 struct Col2D {
     float r[2], c[2], P0[2];   // h_a(P) = fma(P - P0_a, r_a, c_a), a = x, y
@@ -27,8 +34,7 @@ struct Col2D {
     int sg[2];                 // direction signs (0: parallel to the planes)
     int ifix[2];               // index along an axis with sg == 0
     float hs, hd;              // source and pixel positions along h
-    double hc, lxy;            // closest-approach parameter, horizontal length
-    double sz, qz0, vz;        // source z, pixel z at kv = 0, z step per row
+    ColZ z;                    // what the rows of the column need
 };
 
 // Per-row z walk.
@@ -49,11 +55,11 @@ __device__ inline bool col_setup(const ViewRays& vr, const int lo[3], const int 
     if (!(l > 0.0)) return false;
     const double u[2] = {dx / l, dy / l};
     const double hc = -(vr.S[0] * u[0] + vr.S[1] * u[1]);
-    C.hc = hc;
-    C.lxy = l;
-    C.sz = vr.S[2];
-    C.qz0 = vr.Q00[2] + (double)ju * vr.U[2];
-    C.vz = vr.V[2];
+    C.z.hc = hc;
+    C.z.il = 1.0 / l;
+    C.z.sz = vr.S[2];
+    C.z.qz0 = vr.Q00[2] + (double)ju * vr.U[2];
+    C.z.vz = vr.V[2];
     C.hs = (float)(-hc);
     C.hd = (float)(l - hc);
     for (int a = 0; a < 2; a++) {
@@ -167,15 +173,22 @@ __device__ __forceinline__ int seq_count(const AxisSeq& q, float h, bool strict,
     return k;
 }
 
-// z walk of row kv of column C (FP64 set-up).  Returns false for a ray that
-// cannot meet the z range [zlo, zhi) (horizontal rays outside it).
-__device__ inline bool lane_setup(const Col2D& C, float kv, int zlo, int zhi, LaneZ& Z) {
+// z walk of row kv of a column.  The slope m and the z offset of the
+// row's closest approach, zab, are formed in FP64 (zab carries the crossing
+// positions: an error e in zab moves every crossing by e/m); the plane base
+// P0 = floor(zab) and P0 - zab are rounded once to FP32 and the reciprocal
+// slope is a single FP32 rcp, so a crossing is off by at most a few 1e-7 of
+// the ray length.  Forward and adjoint share this routine, hence identical
+// constants.  Returns false for a ray that cannot meet [zlo, zhi)
+// (horizontal rays outside it).
+__device__ inline bool lane_setup(const ColZ& C, float kv, int zlo, int zhi, LaneZ& Z) {
     const double qz = C.qz0 + (double)kv * C.vz;
-    const double m = (qz - C.sz) / C.lxy;
+    const double m = (qz - C.sz) * C.il;
     const double zab = C.sz + C.hc * m + 0.5 * (double)(zlo + zhi);
-    Z.sec = (float)sqrt(1.0 + m * m);
-    Z.u = (float)m;
-    if (fabs(m) < 1e-20) {
+    const float mf = (float)m;
+    Z.sec = sqrtf(__builtin_fmaf(mf, mf, 1.f));
+    Z.u = mf;
+    if (fabsf(mf) < 1e-20f) {
         Z.sg = 0;
         Z.r = 0.f;
         Z.c = AP_BIG;
@@ -185,10 +198,11 @@ __device__ inline bool lane_setup(const Col2D& C, float kv, int zlo, int zhi, La
         return true;
     }
     const double P0 = floor(zab);
+    const float rm = __builtin_amdgcn_rcpf(mf);
     Z.P0 = (float)P0;
-    Z.sg = m > 0.0 ? 1 : -1;
-    Z.r = (float)(1.0 / m);
-    Z.c = (float)((P0 - zab) / m);
+    Z.sg = mf > 0.f ? 1 : -1;
+    Z.r = rm;
+    Z.c = (float)(P0 - zab) * rm;
     Z.ifix = 0;
     return true;
 }

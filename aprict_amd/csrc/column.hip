@@ -38,14 +38,196 @@ __device__ __forceinline__ float sub_off(uint32_t s, uint32_t ss) {
     return -0.5f + ((float)s + 0.5f) / (float)ss;
 }
 
-template <typename Tv, typename Tp, typename IdxT>
+// Volume element at offset o: bytes for 32-bit offsets, elements for 64-bit.
+template <typename Tv, typename OffT>
+__device__ __forceinline__ float vget(const Tv* __restrict__ v, OffT o) {
+    if constexpr (sizeof(OffT) == 4)
+        return to_f32(*reinterpret_cast<const Tv*>(reinterpret_cast<const char*>(v) + o));
+    else
+        return to_f32(v[o]);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// First index s in [0, n) with he[s] > x (n if none); he is non-decreasing.
+__device__ __forceinline__ int first_above(const float* he, int n, float x) {
+    int a = 0, b = n;
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if (he[m] > x) b = m;
+        else a = m + 1;
+    }
+    return a;
+}
+
+// One row's z walk along segments [s0, s1) of a column list (he: segment
+// end parameters, ob: x-y offsets).  hp starts at the row's entry t0; every
+// z crossing before a segment's clamped end splits that segment.  Weights
+// are differences of the same FP32 values the adjoint uses.
+template <typename Tv, typename OffT>
+__device__ __forceinline__ float walk_rows_general(const Tv* __restrict__ vol, const float* he, const OffT* ob,
+                                                   int s0, int s1, float t0, float t1, float hz, float pz,
+                                                   float gz, float rz, float cz, OffT zoff, OffT dz) {
+    float acc = 0.f, hp = t0;
+    for (int s = s0; s < s1; s++) {
+        const float ce = __builtin_amdgcn_fmed3f(he[s], t0, t1);
+        const OffT o = ob[s];
+        while (hz < ce) {
+            acc = __builtin_fmaf(hz - hp, vget(vol, (OffT)(o + zoff)), acc);
+            hp = hz;
+            pz += gz;
+            hz = tcross(pz, rz, cz);
+            zoff += dz;
+        }
+        acc = __builtin_fmaf(ce - hp, vget(vol, (OffT)(o + zoff)), acc);
+        hp = ce;
+    }
+    return acc;
+}
+
+// The same walk when no segment holds more than one z crossing of any row
+// of the wave: straight-line code over 8 segments at a time, so that 16
+// gathers are in flight per lane; a segment without a crossing adds a zero
+// weight.  The segment list is read with wave-uniform addresses (broadcast).
+template <typename Tv>
+__device__ __forceinline__ float walk_rows_single(const Tv* __restrict__ vol, const float* he, const uint32_t* ob,
+                                                  int s0, int s1, float t0, float t1, float hz, float pz,
+                                                  float gz, float rz, float cz, uint32_t zoff, uint32_t dz) {
+    float acc = 0.f, hp = t0;
+    for (int s = s0 & ~3; s < s1; s += 8) {
+        const float4 h0 = *reinterpret_cast<const float4*>(he + s);
+        const float4 h1 = *reinterpret_cast<const float4*>(he + s + 4);
+        const uint4 o0 = *reinterpret_cast<const uint4*>(ob + s);
+        const uint4 o1 = *reinterpret_cast<const uint4*>(ob + s + 4);
+        const float hv[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+        const uint32_t ov[8] = {o0.x, o0.y, o0.z, o0.w, o1.x, o1.y, o1.z, o1.w};
+        float wb[8], wa[8], vb[8], va[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const float ce = __builtin_amdgcn_fmed3f(hv[q], t0, t1);
+            const bool cr = hz < ce;
+            const float c = cr ? hz : ce;
+            wb[q] = c - hp;
+            wa[q] = ce - c;
+            hp = ce;
+            const uint32_t a0 = ov[q] + zoff;
+            zoff += cr ? dz : 0u;
+            pz += cr ? gz : 0.f;
+            hz = tcross(pz, rz, cz);
+            vb[q] = vget(vol, a0);
+            va[q] = vget(vol, ov[q] + zoff);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            acc = __builtin_fmaf(wb[q], vb[q], acc);
+            acc = __builtin_fmaf(wa[q], va[q], acc);
+        }
+    }
+    return acc;
+}
+
+// Rows of a wave whose z slope |m| stays below this never cross two z planes
+// inside one x-y segment: segments are at most sqrt(2) long in h and z planes
+// are 1/|m| apart.
+constexpr float kSingleSlope = 0.6f;
+
+// Merged x-y crossing list of column C inside the box [lo2, hi2), built by
+// the nthr threads of the block: seg_he[s] is the end parameter of segment s,
+// seg_ob[s] the x-y offset of its voxel; 8 zero-length padding segments
+// follow the last one.  Returns the number of segments.  The caller
+// synchronises before (the previous list is consumed) and after.
+template <typename OffT>
+__device__ inline int build_column_list(const Col2D& C, float h0, float h1, const int lo2[2], const int hi2[2],
+                                        OffT vsx, OffT vsy, float* seg_he, OffT* seg_ob, int tid, int nthr) {
+    const int ix0 = axis_index_at(C.r[0], C.c[0], C.P0[0], C.u[0], C.sg[0], C.ifix[0], h0, lo2[0], hi2[0]);
+    const int iy0 = axis_index_at(C.r[1], C.c[1], C.P0[1], C.u[1], C.sg[1], C.ifix[1], h0, lo2[1], hi2[1]);
+    const AxisSeq qx = axis_seq(C, 0, ix0, h1, lo2[0], hi2[0]);
+    const AxisSeq qy = axis_seq(C, 1, iy0, h1, lo2[1], hi2[1]);
+    const int nseg = qx.n + qy.n + 1;
+    const float aux = fabsf(C.u[0]), auy = fabsf(C.u[1]);
+    // segment R-1 ends at the crossing of rank R; segment R starts there
+    for (int m = tid; m < qx.n; m += nthr) {
+        const float h = seq_at(qx, m);
+        const int cy = seq_count(qy, h, true, auy);
+        const int R = m + cy + 1;
+        const int ix = min(max(ix0 + qx.sg * (m + 1), lo2[0]), hi2[0] - 1);
+        const int iy = min(max(iy0 + qy.sg * cy, lo2[1]), hi2[1] - 1);
+        seg_he[R - 1] = h;
+        seg_ob[R] = (OffT)ix * vsx + (OffT)iy * vsy;
+    }
+    for (int l = tid; l < qy.n; l += nthr) {
+        const float h = seq_at(qy, l);
+        const int cx = seq_count(qx, h, false, aux);
+        const int R = l + cx + 1;
+        const int ix = min(max(ix0 + qx.sg * cx, lo2[0]), hi2[0] - 1);
+        const int iy = min(max(iy0 + qy.sg * (l + 1), lo2[1]), hi2[1] - 1);
+        seg_he[R - 1] = h;
+        seg_ob[R] = (OffT)ix * vsx + (OffT)iy * vsy;
+    }
+    if (tid == 0) seg_ob[0] = (OffT)ix0 * vsx + (OffT)iy0 * vsy;
+    if (tid < 9) {
+        // the last segment ends at h1; the padding repeats the final voxel so
+        // the unrolled walks may read up to 7 entries past the end
+        const int ixe = min(max(ix0 + qx.sg * qx.n, lo2[0]), hi2[0] - 1);
+        const int iye = min(max(iy0 + qy.sg * qy.n, lo2[1]), hi2[1] - 1);
+        seg_he[nseg - 1 + tid] = h1;
+        if (tid > 0) seg_ob[nseg - 1 + tid] = (OffT)ixe * vsx + (OffT)iye * vsy;
+    }
+    return nseg;
+}
+
+// z walk state of one row of a column: entry t0 / exit t1 of the chunk, the
+// voxel index at t0 and the next z crossing.  Inactive rows get an empty
+// range at h0 and an in-range index, so they may run the walks unmasked.
+struct RowWalk {
+    float t0, t1, hz, pz, gz, rz, cz, sec, slope;
+    int iz, sg;
+    bool act;
+};
+
+__device__ inline RowWalk row_setup(const Col2D& C, float h0, float h1, float kv, bool in, int zlo, int zhi) {
+    RowWalk W;
+    W.t0 = h0; W.t1 = h0; W.hz = AP_BIG; W.pz = 0.f; W.gz = 0.f; W.rz = 0.f; W.cz = AP_BIG;
+    W.sec = 0.f; W.slope = 0.f; W.iz = zlo; W.sg = 0; W.act = false;
+    LaneZ Z;
+    if (!in || !lane_setup(C.z, kv, zlo, zhi, Z)) return W;
+    float zin, zout;
+    lane_box(Z, zlo, zhi, zin, zout);
+    const float lin = fmaxf(h0, zin), lout = fminf(h1, zout);
+    if (!(lin < lout)) return W;
+    W.act = true;
+    W.t0 = lin;
+    W.t1 = lout;
+    W.sec = Z.sec;
+    W.slope = fabsf(Z.u);
+    W.sg = Z.sg;
+    W.iz = axis_index_at(Z.r, Z.c, Z.P0, Z.u, Z.sg, Z.ifix, lin, zlo, zhi);
+    if (Z.sg != 0) {
+        W.pz = Z.sg > 0 ? (float)(W.iz + 1) - Z.P0 : (float)W.iz - Z.P0;
+        W.gz = (float)Z.sg;
+        W.rz = Z.r;
+        W.cz = Z.c;
+        W.hz = tcross(W.pz, W.rz, W.cz);
+    }
+    return W;
+}
+
+template <typename Tv, typename Tp, typename OffT>
 __global__ __launch_bounds__(kRowsF) void k_project_columns(
     RayCfg rc, SinoCfgC sc, ColMap cm, float mba, uint32_t ssu, uint32_t ssv,
-    const ap_proj_spec* __restrict__ specs, const Tv* __restrict__ vol, IdxT vsx, IdxT vsy, IdxT vsz,
+    const ap_proj_spec* __restrict__ specs, const Tv* __restrict__ vol, OffT vsx, OffT vsy, OffT vsz,
     Tp* __restrict__ sino, const Tp* __restrict__ b, uint32_t cap) {
     extern __shared__ __align__(16) char smem[];
-    float* seg_h = reinterpret_cast<float*>(smem);
-    IdxT* seg_o = reinterpret_cast<IdxT*>(smem + ((cap * sizeof(float) + 15) & ~size_t(15)));
+    float* seg_he = reinterpret_cast<float*>(smem);
+    OffT* seg_ob = reinterpret_cast<OffT*>(smem + ((cap * sizeof(float) + 15) & ~size_t(15)));
     const uint32_t L = blockIdx.x;
     const uint32_t logical = (L % kXcdC) * cm.per_xcd + L / kXcdC;
     if (logical >= cm.total) return;
@@ -71,70 +253,33 @@ __global__ __launch_bounds__(kRowsF) void k_project_columns(
             ok = h0 < h1;
         }
         if (!ok) continue;   // block-uniform
-        const int ix0 = axis_index_at(C.r[0], C.c[0], C.P0[0], C.u[0], C.sg[0], C.ifix[0], h0, lo2[0], hi2[0]);
-        const int iy0 = axis_index_at(C.r[1], C.c[1], C.P0[1], C.u[1], C.sg[1], C.ifix[1], h0, lo2[1], hi2[1]);
-        const AxisSeq qx = axis_seq(C, 0, ix0, h1, lo2[0], hi2[0]);
-        const AxisSeq qy = axis_seq(C, 1, iy0, h1, lo2[1], hi2[1]);
-        const int nseg = qx.n + qy.n + 1;
-        const float aux = fabsf(C.u[0]), auy = fabsf(C.u[1]);
         __syncthreads();   // the previous sub-column's list is consumed
-        for (int m = tid; m < qx.n; m += kRowsF) {
-            const float h = seq_at(qx, m);
-            const int cy = seq_count(qy, h, true, auy);
-            const int R = m + cy + 1;
-            const int ix = min(max(ix0 + qx.sg * (m + 1), lo2[0]), hi2[0] - 1);
-            const int iy = min(max(iy0 + qy.sg * cy, lo2[1]), hi2[1] - 1);
-            seg_h[R] = h;
-            seg_o[R] = (IdxT)ix * vsx + (IdxT)iy * vsy;
-        }
-        for (int l = tid; l < qy.n; l += kRowsF) {
-            const float h = seq_at(qy, l);
-            const int cx = seq_count(qx, h, false, aux);
-            const int R = l + cx + 1;
-            const int ix = min(max(ix0 + qx.sg * cx, lo2[0]), hi2[0] - 1);
-            const int iy = min(max(iy0 + qy.sg * (l + 1), lo2[1]), hi2[1] - 1);
-            seg_h[R] = h;
-            seg_o[R] = (IdxT)ix * vsx + (IdxT)iy * vsy;
-        }
-        if (tid == 0) {
-            seg_h[0] = h0;
-            seg_o[0] = (IdxT)ix0 * vsx + (IdxT)iy0 * vsy;
-            seg_h[nseg] = h1;
-        }
+        const int nseg = build_column_list(C, h0, h1, lo2, hi2, vsx, vsy, seg_he, seg_ob, tid, kRowsF);
         __syncthreads();
-        if (!in) continue;
         float accu = 0.f;
         for (uint32_t sv = 0; sv < ssv; sv++) {
-            LaneZ Z;
-            if (!lane_setup(C, (float)k + sub_off(sv, ssv), rc.lo[2], rc.hi[2], Z)) continue;
-            float zin, zout;
-            lane_box(Z, rc.lo[2], rc.hi[2], zin, zout);
-            const float lin = fmaxf(h0, zin), lout = fminf(h1, zout);
-            if (!(lin < lout)) continue;
-            const int iz = axis_index_at(Z.r, Z.c, Z.P0, Z.u, Z.sg, Z.ifix, lin, rc.lo[2], rc.hi[2]);
-            float pz = Z.sg > 0 ? (float)(iz + 1) - Z.P0 : (float)iz - Z.P0;
-            float hz = Z.sg == 0 ? AP_BIG : tcross(pz, Z.r, Z.c);
-            const float gz = (float)Z.sg;
-            IdxT zoff = (IdxT)iz * vsz;
-            const IdxT dz = Z.sg >= 0 ? vsz : -vsz;
-            float acc = 0.f;
-            float hprev = h0;
-            for (int s = 0; s < nseg; s++) {
-                const float hn = seg_h[s + 1];
-                const IdxT o = seg_o[s];
-                float a = fmaxf(hprev, lin);
-                const float bnd = fminf(hn, lout);
-                while (hz < bnd) {
-                    acc = __builtin_fmaf(fmaxf(hz - a, 0.f), to_f32(vol[o + zoff]), acc);
-                    a = fmaxf(a, hz);
-                    pz += gz;
-                    hz = tcross(pz, Z.r, Z.c);
-                    zoff += dz;
-                }
-                acc = __builtin_fmaf(fmaxf(bnd - a, 0.f), to_f32(vol[o + zoff]), acc);
-                hprev = hn;
+            const RowWalk W = row_setup(C, h0, h1, (float)k + sub_off(sv, ssv), in, rc.lo[2], rc.hi[2]);
+            const float wt0 = wave_min(W.act ? W.t0 : AP_BIG);
+            const float wt1 = wave_max(W.act ? W.t1 : -AP_BIG);
+            if (!(wt0 < wt1)) continue;   // wave-uniform
+            const int s0 = first_above(seg_he, nseg, wt0);
+            const int s1 = min(first_above(seg_he, nseg, wt1) + 1, nseg);
+            const bool steep = __ballot(W.act && W.slope >= kSingleSlope) != 0;
+            const OffT zoff = (OffT)W.iz * vsz;
+            const OffT dz = W.sg >= 0 ? vsz : (OffT)(OffT(0) - vsz);
+            float acc;
+            if constexpr (sizeof(OffT) == 4) {
+                if (!steep)
+                    acc = walk_rows_single(vol, seg_he, seg_ob, s0, s1, W.t0, W.t1, W.hz, W.pz, W.gz, W.rz, W.cz,
+                                           zoff, dz);
+                else
+                    acc = walk_rows_general(vol, seg_he, seg_ob, s0, s1, W.t0, W.t1, W.hz, W.pz, W.gz, W.rz, W.cz,
+                                            zoff, dz);
+            } else {
+                acc = walk_rows_general(vol, seg_he, seg_ob, s0, s1, W.t0, W.t1, W.hz, W.pz, W.gz, W.rz, W.cz, zoff,
+                                        dz);
             }
-            accu += acc * Z.sec;
+            if (W.act) accu += acc * W.sec;
         }
         tot += ssv > 1 ? accu / (float)ssv : accu;
     }
@@ -147,16 +292,45 @@ __global__ __launch_bounds__(kRowsF) void k_project_columns(
 }
 
 // ------------------------------------------------------------------ adjoint
-constexpr int kTX = 16, kTY = 16, kTZ = 64;
-constexpr int kAdjT = 256;
-constexpr int kWaves = kAdjT / 64;
-constexpr int kVB = 16;            // views prepared per batch
-constexpr int kSegCap = kTX + kTY + 2;
+// A workgroup of kAdjT threads owns a brick of kTX x kTY x kTZ voxels
+// accumulated in LDS as 64-bit fixed point: float atomic adds in LDS run at a
+// small fraction of the rate of integer ones, and integer sums do not depend
+// on the order of the adds, so the result is deterministic.  The fixed-point
+// scale S is set per workgroup from the largest |y| over the brick's detector
+// footprints, so that every term rint(w * y * scale * sec * S) fits in 31 bits
+// and the 64-bit sums cannot overflow (views x rays per view < 2^33, checked
+// by the host); the quantisation error is about 1e-9 of the largest term.
+//
+// Each wave processes its own views, one after another, without workgroup
+// barriers: it projects the brick's corners to find the detector footprint,
+// sets up up to kWCols sub-columns at a time (2-D line, crossing sequences,
+// merged segment list inside the brick, all in the wave's own LDS area), then
+// every lane takes (sub-column, sub-row) pairs and scatters along its ray.
+// Segment boundaries and weights are those of the forward walk.
+constexpr int kTX = 16, kTY = 16, kTZ = 32;
+constexpr int kAdjT = 1024;
+constexpr int kAdjWaves = kAdjT / 64;
+constexpr int kWCols = 8;               // sub-columns per wave pass
+// segments of one column inside a brick (<= kTX + kTY - 1) plus 3 padding
+// entries, rounded to whole float4 rows
+constexpr int kListCap = kTX + kTY + 4;
 
-struct BrickViewC {
-    ViewRays vr;
-    float j0f, j1f;                // column range (fractional, before clamping)
-    int j0, j1, k0, k1;
+// What the rows of one sub-column need, and the state that builds its list.
+struct ColEntry {
+    ColZ z;
+    float h0, h1;                  // the column line inside the brick
+    int nseg;                      // 0: the column misses the brick
+    int j;
+    AxisSeq qx, qy;
+    int ix0, iy0;
+    float aux, auy;
+};
+
+struct WaveArea {
+    ViewRays vr;                   // the current view (kept here, not in registers)
+    ColEntry ce[kWCols];
+    float he[kWCols][kListCap];
+    int ob[kWCols][kListCap];
 };
 
 // pixel coordinates of the central projection (float), see project.hip
@@ -194,16 +368,51 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Detector footprint [j0, j1] x [k0, k1] of the brick [blo, bhi) in view vr
+// (wave-uniform; lanes 0-7 project one corner each).  Returns false when it
+// is empty.
+__device__ inline bool brick_footprint(const ViewRays& vr, const int blo[3], const int bhi[3], const float cen[3],
+                                       uint32_t nu, uint32_t nv, int lane, int& j0, int& j1, int& k0, int& k1) {
+    float jmn = AP_BIG, jmx = -AP_BIG, kmn = AP_BIG, kmx = -AP_BIG, bad = 0.f;
+    if (lane < 8) {
+        const float X[3] = {(float)((lane & 4) ? bhi[0] : blo[0]) - cen[0],
+                            (float)((lane & 2) ? bhi[1] : blo[1]) - cen[1],
+                            (float)((lane & 1) ? bhi[2] : blo[2]) - cen[2]};
+        float ju, kv;
+        if (proj_pt(vr, X, ju, kv)) {
+            jmn = jmx = ju;
+            kmn = kmx = kv;
+        } else {
+            bad = 1.f;
+        }
+    }
+    jmn = wave_min(jmn); jmx = wave_max(jmx);
+    kmn = wave_min(kmn); kmx = wave_max(kmx);
+    if (wave_max(bad) > 0.f) {
+        j0 = 0; j1 = (int)nu - 1; k0 = 0; k1 = (int)nv - 1;
+    } else {
+        j0 = max(0, (int)floorf(jmn - 0.501f));
+        j1 = min((int)nu - 1, (int)ceilf(jmx + 0.501f));
+        k0 = max(0, (int)floorf(kmn - 0.501f));
+        k1 = min((int)nv - 1, (int)ceilf(kmx + 0.501f));
+    }
+    return j0 <= j1 && k0 <= k1;
+}
+
+// One fixed-point term: w * wys rounded to an integer (|w * wys| < 2^30).
+__device__ __forceinline__ unsigned long long fx_term(float w, float wys) {
+    return (unsigned long long)(long long)__float2int_rn(w * wys);
+}
+
 template <typename Tv, typename Tp, typename Tm, typename IdxT>
 __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
     RayCfg rc, SinoCfgC sc, float scale, uint32_t ssu, uint32_t ssv,
     const ap_proj_spec* __restrict__ specs, Tv* __restrict__ vol, IdxT vsx, IdxT vsy, IdxT vsz,
     const Tm* __restrict__ mult, IdxT msx, IdxT msy, IdxT msz, const Tp* __restrict__ sino,
     uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t per_xcd) {
-    __shared__ float acc[kTX * kTY * kTZ];
-    __shared__ BrickViewC bv[kVB];
-    __shared__ float seg_h[kWaves][kSegCap];
-    __shared__ int seg_o[kWaves][kSegCap];
+    __shared__ unsigned long long acc[kTX * kTY * kTZ];
+    __shared__ __align__(16) WaveArea wa_all[kAdjWaves];
+    __shared__ float wmax[kAdjWaves];
     const uint32_t L = blockIdx.x;
     const uint32_t logical = (L % kXcdC) * per_xcd + L / kXcdC;
     if (logical >= nbx * nby * nbz) return;
@@ -211,136 +420,223 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
     const int blo[3] = {rc.lo[0] + (int)bi * kTX, rc.lo[1] + (int)bj * kTY, rc.lo[2] + (int)bk * kTZ};
     const int bhi[3] = {min(blo[0] + kTX, rc.hi[0]), min(blo[1] + kTY, rc.hi[1]), min(blo[2] + kTZ, rc.hi[2])};
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int e = tid; e < kTX * kTY * kTZ; e += kAdjT) acc[e] = 0.f;
+    WaveArea& W = wa_all[wave];
     const float cen[3] = {0.5f * (float)(rc.lo[0] + rc.hi[0]), 0.5f * (float)(rc.lo[1] + rc.hi[1]),
                           0.5f * (float)(rc.lo[2] + rc.hi[2])};
-    float* my_h = seg_h[wave];
-    int* my_o = seg_o[wave];
-    for (uint32_t p0 = 0; p0 < sc.V; p0 += kVB) {
-        const uint32_t nb = min((uint32_t)kVB, sc.V - p0);
-        __syncthreads();
-        if (tid < (int)nb) {
-            BrickViewC& B = bv[tid];
-            const uint32_t p = p0 + tid;
-            B.j0 = 0; B.j1 = -1; B.k0 = 0; B.k1 = -1;
-            if (view_is_upright(specs, p)) {
-                B.vr = make_view_rays(rc, specs, p);
-                float jmin = AP_BIG, jmax = -AP_BIG, kmin = AP_BIG, kmax = -AP_BIG;
-                bool okp = true;
-                for (int q = 0; q < 8; q++) {
-                    const float X[3] = {(float)((q & 4) ? bhi[0] : blo[0]) - cen[0],
-                                        (float)((q & 2) ? bhi[1] : blo[1]) - cen[1],
-                                        (float)((q & 1) ? bhi[2] : blo[2]) - cen[2]};
-                    float ju, kv;
-                    if (!proj_pt(B.vr, X, ju, kv)) { okp = false; break; }
-                    jmin = fminf(jmin, ju); jmax = fmaxf(jmax, ju);
-                    kmin = fminf(kmin, kv); kmax = fmaxf(kmax, kv);
-                }
-                if (okp) {
-                    B.j0 = max(0, (int)floorf(jmin - 0.501f));
-                    B.j1 = min((int)sc.nu - 1, (int)ceilf(jmax + 0.501f));
-                    B.k0 = max(0, (int)floorf(kmin - 0.501f));
-                    B.k1 = min((int)sc.nv - 1, (int)ceilf(kmax + 0.501f));
-                } else {
-                    B.j0 = 0; B.j1 = (int)sc.nu - 1; B.k0 = 0; B.k1 = (int)sc.nv - 1;
+    // pass 1: largest |y| over the brick's footprints sets the fixed-point scale
+    float ymax = 0.f;
+    for (uint32_t p = wave; p < sc.V; p += kAdjWaves) {
+        if (!view_is_upright(specs, p)) continue;   // wave-uniform
+        int j0, j1, k0, k1;
+        if (!brick_footprint(make_view_rays(rc, specs, p), blo, bhi, cen, sc.nu, sc.nv, lane, j0, j1, k0, k1))
+            continue;
+        const int nk = k1 - k0 + 1, npix = (j1 - j0 + 1) * nk;
+        for (int e = lane; e < npix; e += 64) {
+            const int jj = e / nk;
+            const float y = to_f32(sino[(int64_t)p * sc.s0 + (int64_t)(j0 + jj) * sc.s1 +
+                                        (int64_t)(k0 + e - jj * nk) * sc.s2]);
+            ymax = fmaxf(ymax, fabsf(y));   // a NaN sample is caught below
+            if (y != y) ymax = __builtin_inff();
+        }
+    }
+    ymax = wave_max(ymax);
+    if (lane == 0) wmax[wave] = ymax;
+    for (int e = tid; e < kTX * kTY * kTZ; e += kAdjT) acc[e] = 0ull;
+    __syncthreads();
+    ymax = 0.f;
+    for (int w = 0; w < kAdjWaves; w++) ymax = fmaxf(ymax, wmax[w]);
+    const float amax = ymax * fabsf(scale);
+    if (amax == 0.f) return;   // block-uniform: nothing to add
+    if (!(amax < __builtin_inff())) {
+        // non-finite data: the sum is not representable; propagate NaN
+        for (int e = tid; e < kTX * kTY * kTZ; e += kAdjT) {
+            const int li = e / (kTY * kTZ), lj = (e / kTZ) % kTY, lk = e % kTZ;
+            const int gi = blo[0] + li, gj = blo[1] + lj, gk = blo[2] + lk;
+            if (gi >= bhi[0] || gj >= bhi[1] || gk >= bhi[2]) continue;
+            vol[(IdxT)gi * vsx + (IdxT)gj * vsy + (IdxT)gk * vsz] = from_f32<Tv>(__builtin_nanf(""));
+        }
+        return;
+    }
+    // |w * sec| <= sqrt(3) < 2, so every term is below 2^30 in magnitude
+    const float S = 0x1p29f / amax;
+    const float scaleS = scale * S;
+    for (uint32_t p = wave; p < sc.V; p += kAdjWaves) {
+        if (!view_is_upright(specs, p)) continue;   // wave-uniform
+        wave_sync_lds();   // the previous view is done with the area
+        if (lane == 0) W.vr = make_view_rays(rc, specs, p);
+        wave_sync_lds();
+        int j0, j1, k0, k1;
+        if (!brick_footprint(W.vr, blo, bhi, cen, sc.nu, sc.nv, lane, j0, j1, k0, k1)) continue;
+        const int ncol = (j1 - j0 + 1) * (int)ssu;
+        const int nrow = (k1 - k0 + 1) * (int)ssv;
+        for (int c0 = 0; c0 < ncol; c0 += kWCols) {
+            const int nc = min(kWCols, ncol - c0);
+            wave_sync_lds();   // the previous pass is done with the tables
+            if (lane < nc) {
+                ColEntry& E = W.ce[lane];
+                const int g = c0 + lane;
+                E.j = j0 + g / (int)ssu;
+                E.nseg = 0;
+                E.qx.n = 0;   // the list builder reads the counts of every column
+                E.qy.n = 0;
+                Col2D C;
+                if (col_setup(W.vr, rc.lo, rc.hi, (float)E.j + sub_off((uint32_t)(g % (int)ssu), ssu), C)) {
+                    float h0, h1;
+                    col_box(C, blo, bhi, h0, h1);
+                    if (h0 < h1) {
+                        E.z = C.z;
+                        E.h0 = h0;
+                        E.h1 = h1;
+                        E.ix0 = axis_index_at(C.r[0], C.c[0], C.P0[0], C.u[0], C.sg[0], C.ifix[0], h0, blo[0],
+                                              bhi[0]);
+                        E.iy0 = axis_index_at(C.r[1], C.c[1], C.P0[1], C.u[1], C.sg[1], C.ifix[1], h0, blo[1],
+                                              bhi[1]);
+                        E.qx = axis_seq(C, 0, E.ix0, h1, blo[0], bhi[0]);
+                        E.qy = axis_seq(C, 1, E.iy0, h1, blo[1], bhi[1]);
+                        E.aux = fabsf(C.u[0]);
+                        E.auy = fabsf(C.u[1]);
+                        E.nseg = E.qx.n + E.qy.n + 1;
+                        W.ob[lane][0] = ((E.ix0 - blo[0]) * kTY + (E.iy0 - blo[1])) * kTZ;
+                        // the last segment ends at h1; 3 zero-length padding
+                        // segments repeat the final cell for the 4-wide walk
+                        const int ixe = min(max(E.ix0 + E.qx.sg * E.qx.n, blo[0]), bhi[0] - 1);
+                        const int iye = min(max(E.iy0 + E.qy.sg * E.qy.n, blo[1]), bhi[1] - 1);
+                        const int oe = ((ixe - blo[0]) * kTY + (iye - blo[1])) * kTZ;
+                        for (int q = 0; q < 4; q++) W.he[lane][E.nseg - 1 + q] = h1;
+                        for (int q = 1; q < 4; q++) W.ob[lane][E.nseg - 1 + q] = oe;
+                    }
                 }
             }
-        }
-        __syncthreads();
-        for (uint32_t q = 0; q < nb; q++) {
-            const BrickViewC& B = bv[q];
-            if (B.j1 < B.j0 || B.k1 < B.k0) continue;
-            const uint32_t p = p0 + q;
-            const int ncol = (B.j1 - B.j0 + 1) * (int)ssu;
-            const int nchunk = (B.k1 - B.k0 + 64) / 64;
-            const int nunits = ncol * nchunk;
-            for (int un = wave; un < nunits; un += kWaves) {
-                const int colx = un / nchunk, chunk = un % nchunk;
-                const int j = B.j0 + colx / (int)ssu;
-                const uint32_t su = (uint32_t)(colx % (int)ssu);
-                Col2D C;
-                if (!col_setup(B.vr, rc.lo, rc.hi, (float)j + sub_off(su, ssu), C)) continue;
-                float h0, h1;
-                col_box(C, blo, bhi, h0, h1);
-                // the ray segment must also lie inside the chunk (it does: brick within chunk)
-                if (!(h0 < h1)) continue;
-                const int ix0 = axis_index_at(C.r[0], C.c[0], C.P0[0], C.u[0], C.sg[0], C.ifix[0], h0, blo[0], bhi[0]);
-                const int iy0 = axis_index_at(C.r[1], C.c[1], C.P0[1], C.u[1], C.sg[1], C.ifix[1], h0, blo[1], bhi[1]);
-                const AxisSeq qx = axis_seq(C, 0, ix0, h1, blo[0], bhi[0]);
-                const AxisSeq qy = axis_seq(C, 1, iy0, h1, blo[1], bhi[1]);
-                const int nseg = qx.n + qy.n + 1;
-                wave_sync_lds();   // previous unit's list fully read
-                if (lane < qx.n) {
-                    const float h = seq_at(qx, lane);
-                    const int cy = seq_count(qy, h, true, fabsf(C.u[1]));
-                    const int ix = min(max(ix0 + qx.sg * (lane + 1), blo[0]), bhi[0] - 1);
-                    const int iy = min(max(iy0 + qy.sg * cy, blo[1]), bhi[1] - 1);
-                    my_h[lane + cy + 1] = h;
-                    my_o[lane + cy + 1] = ((ix - blo[0]) * kTY + (iy - blo[1])) * kTZ;
+            wave_sync_lds();
+            // merged x-y crossing lists: (column, crossing) pairs across the wave
+            for (int e = lane; e < nc * kTX; e += 64) {
+                const int g = e / kTX, m = e % kTX;
+                const ColEntry& E = W.ce[g];
+                if (m < E.qx.n) {
+                    const float h = seq_at(E.qx, m);
+                    const int cy = seq_count(E.qy, h, true, E.auy);
+                    const int ix = min(max(E.ix0 + E.qx.sg * (m + 1), blo[0]), bhi[0] - 1);
+                    const int iy = min(max(E.iy0 + E.qy.sg * cy, blo[1]), bhi[1] - 1);
+                    W.he[g][m + cy] = h;
+                    W.ob[g][m + cy + 1] = ((ix - blo[0]) * kTY + (iy - blo[1])) * kTZ;
                 }
-                if (lane < qy.n) {
-                    const float h = seq_at(qy, lane);
-                    const int cx = seq_count(qx, h, false, fabsf(C.u[0]));
-                    const int ix = min(max(ix0 + qx.sg * cx, blo[0]), bhi[0] - 1);
-                    const int iy = min(max(iy0 + qy.sg * (lane + 1), blo[1]), bhi[1] - 1);
-                    my_h[lane + cx + 1] = h;
-                    my_o[lane + cx + 1] = ((ix - blo[0]) * kTY + (iy - blo[1])) * kTZ;
+                if (m < E.qy.n) {
+                    const float h = seq_at(E.qy, m);
+                    const int cx = seq_count(E.qx, h, false, E.aux);
+                    const int ix = min(max(E.ix0 + E.qx.sg * cx, blo[0]), bhi[0] - 1);
+                    const int iy = min(max(E.iy0 + E.qy.sg * (m + 1), blo[1]), bhi[1] - 1);
+                    W.he[g][m + cx] = h;
+                    W.ob[g][m + cx + 1] = ((ix - blo[0]) * kTY + (iy - blo[1])) * kTZ;
                 }
-                if (lane == 0) {
-                    my_h[0] = h0;
-                    my_o[0] = ((ix0 - blo[0]) * kTY + (iy0 - blo[1])) * kTZ;
-                    my_h[nseg] = h1;
-                }
-                wave_sync_lds();
-                const int k = B.k0 + chunk * 64 + lane;
-                if (k > B.k1) continue;
-                const float y = to_f32(sino[(int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2]);
-                if (y == 0.f) continue;
-                for (uint32_t sv = 0; sv < ssv; sv++) {
-                    LaneZ Z;
-                    if (!lane_setup(C, (float)k + sub_off(sv, ssv), rc.lo[2], rc.hi[2], Z)) continue;
-                    float zin, zout;
-                    lane_box(Z, blo[2], bhi[2], zin, zout);
-                    const float lin = fmaxf(h0, zin), lout = fminf(h1, zout);
-                    if (!(lin < lout)) continue;
-                    const int iz = axis_index_at(Z.r, Z.c, Z.P0, Z.u, Z.sg, Z.ifix, lin, blo[2], bhi[2]);
-                    float pz = Z.sg > 0 ? (float)(iz + 1) - Z.P0 : (float)iz - Z.P0;
-                    float hz = Z.sg == 0 ? AP_BIG : tcross(pz, Z.r, Z.c);
-                    const float gz = (float)Z.sg;
-                    int zl = iz - blo[2];
-                    const float wy = y * scale * Z.sec;
-                    float hprev = h0;
-                    for (int s = 0; s < nseg; s++) {
-                        const float hn = my_h[s + 1];
-                        const int o = my_o[s];
-                        float a = fmaxf(hprev, lin);
-                        const float bnd = fminf(hn, lout);
-                        while (hz < bnd) {
-                            const float w = fmaxf(hz - a, 0.f);
-                            if (w > 0.f) atomicAdd(&acc[o + zl], w * wy);
-                            a = fmaxf(a, hz);
+            }
+            wave_sync_lds();
+            // (sub-column, sub-row) pairs: every lane scatters one ray at a time;
+            // the sinogram value of the lane's next pair is fetched one pair ahead
+            const int npair = nc * nrow;
+            auto yval = [&](int e) -> float {
+                if (e >= npair) return 0.f;
+                const int g = e / nrow;
+                const int k = k0 + (e - g * nrow) / (int)ssv;
+                return to_f32(sino[(int64_t)p * sc.s0 + (int64_t)W.ce[g].j * sc.s1 + (int64_t)k * sc.s2]);
+            };
+            float y_next = yval(lane);
+            for (int e = lane; e < npair; e += 64) {
+                const float y = y_next;
+                y_next = yval(e + 64);
+                const int g = e / nrow, kk = e - g * nrow;
+                const ColEntry& E = W.ce[g];
+                if (E.nseg == 0 || y == 0.f) continue;
+                const int k = k0 + kk / (int)ssv;
+                LaneZ Z;
+                if (!lane_setup(E.z, (float)k + sub_off((uint32_t)(kk % (int)ssv), ssv), rc.lo[2], rc.hi[2], Z))
+                    continue;
+                float zin, zout;
+                lane_box(Z, blo[2], bhi[2], zin, zout);
+                const float t0 = fmaxf(E.h0, zin), t1 = fminf(E.h1, zout);
+                if (!(t0 < t1)) continue;
+                const int iz = axis_index_at(Z.r, Z.c, Z.P0, Z.u, Z.sg, Z.ifix, t0, blo[2], bhi[2]);
+                float pz = Z.sg > 0 ? (float)(iz + 1) - Z.P0 : (float)iz - Z.P0;
+                float hz = Z.sg == 0 ? AP_BIG : tcross(pz, Z.r, Z.c);
+                const float gz = (float)Z.sg;
+                int zl = iz - blo[2];
+                const float wys = y * scaleS * Z.sec;
+                const float* he = W.he[g];
+                const int* ob = W.ob[g];
+                int s0 = 0;
+                while (he[s0] <= t0) s0++;   // he[nseg-1] = h1 > t0
+                float hp = t0;
+                if (Z.u > -kSingleSlope && Z.u < kSingleSlope) {
+                    // at most one z crossing per segment: 4 segments per step
+                    // (LDS byte offsets), ending with the chunk that reaches t1;
+                    // the add after a crossing is skipped where there is none
+                    char* accb = reinterpret_cast<char*>(acc);
+                    int zb = zl * 8;
+                    const int dzb = Z.sg * 8;
+                    for (int s = s0 & ~3;; s += 4) {
+                        const float4 h4 = *reinterpret_cast<const float4*>(he + s);
+                        const int4 o4 = *reinterpret_cast<const int4*>(ob + s);
+                        const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
+                        const int ov[4] = {o4.x * 8, o4.y * 8, o4.z * 8, o4.w * 8};
+                        // weights and addresses first, then the adds: no LDS
+                        // read is outstanding while the atomics issue
+                        float wb[4], wa[4];
+                        int a0[4], a1[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const float ce = __builtin_amdgcn_fmed3f(hv[q], t0, t1);
+                            const bool cr = hz < ce;
+                            const float c = cr ? hz : ce;
+                            wb[q] = c - hp;
+                            wa[q] = ce - c;
+                            hp = ce;
+                            a0[q] = ov[q] + zb;
+                            zb += cr ? dzb : 0;
+                            a1[q] = ov[q] + zb;
+                            pz += cr ? gz : 0.f;
+                            hz = tcross(pz, Z.r, Z.c);
+                        }
+                        // keep the compiler from sinking the address arithmetic
+                        // (which waits on the list read) between the adds
+                        asm volatile("" : "+v"(a0[0]), "+v"(a0[1]), "+v"(a0[2]), "+v"(a0[3]), "+v"(a1[0]),
+                                     "+v"(a1[1]), "+v"(a1[2]), "+v"(a1[3]));
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            atomicAdd(reinterpret_cast<unsigned long long*>(accb + a0[q]), fx_term(wb[q], wys));
+                            if (wa[q] != 0.f)
+                                atomicAdd(reinterpret_cast<unsigned long long*>(accb + a1[q]), fx_term(wa[q], wys));
+                        }
+                        if (h4.w >= t1 || s + 4 >= E.nseg) break;
+                    }
+                } else {
+                    for (int s = s0; s < E.nseg; s++) {
+                        const float cend = __builtin_amdgcn_fmed3f(he[s], t0, t1);
+                        const int o = ob[s];
+                        while (hz < cend) {
+                            atomicAdd(&acc[o + zl], fx_term(hz - hp, wys));
+                            hp = hz;
                             pz += gz;
                             hz = tcross(pz, Z.r, Z.c);
                             zl += Z.sg;
                         }
-                        const float w = fmaxf(bnd - a, 0.f);
-                        if (w > 0.f) atomicAdd(&acc[o + zl], w * wy);
-                        hprev = hn;
+                        atomicAdd(&acc[o + zl], fx_term(cend - hp, wys));
+                        hp = cend;
+                        if (cend >= t1) break;
                     }
                 }
             }
         }
     }
     __syncthreads();
+    const double inv_s = 1.0 / (double)S;
     for (int e = tid; e < kTX * kTY * kTZ; e += kAdjT) {
         const int li = e / (kTY * kTZ), lj = (e / kTZ) % kTY, lk = e % kTZ;
         const int gi = blo[0] + li, gj = blo[1] + lj, gk = blo[2] + lk;
         if (gi >= bhi[0] || gj >= bhi[1] || gk >= bhi[2]) continue;
-        const float a = acc[e];
-        if (a == 0.f) continue;
+        const long long a = (long long)acc[e];
+        if (a == 0) continue;
         const IdxT ov = (IdxT)gi * vsx + (IdxT)gj * vsy + (IdxT)gk * vsz;
         const float m = mult ? to_f32(mult[(IdxT)gi * msx + (IdxT)gj * msy + (IdxT)gk * msz]) : 1.f;
-        vol[ov] = from_f32<Tv>(__builtin_fmaf(m, a, to_f32(vol[ov])));
+        vol[ov] = from_f32<Tv>(__builtin_fmaf(m, (float)((double)a * inv_s), to_f32(vol[ov])));
     }
 }
 
@@ -353,8 +649,12 @@ int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32
                                int64_t s2, float mba, uint32_t ssu, uint32_t ssv, const ap_proj_spec* specs,
                                const ap_array3* vol_desc, const void* vol, int32_t sino_dtype, void* sino,
                                const void* b, bool idx64, hipStream_t s) {
-    const uint32_t cap = (uint32_t)(rc.hi[0] - rc.lo[0]) + (uint32_t)(rc.hi[1] - rc.lo[1]) + 2;
-    const size_t shm = ((cap * sizeof(float) + 15) & ~size_t(15)) + cap * (idx64 ? 8 : 4);
+    // segment list: at most Nx + Ny + 1 segments plus 8 padding entries
+    const uint32_t cap = ((uint32_t)(rc.hi[0] - rc.lo[0]) + (uint32_t)(rc.hi[1] - rc.lo[1]) + 9 + 3) & ~3u;
+    // 32-bit byte offsets when the whole volume spans less than 4 GiB
+    const uint64_t esz = vol_desc->dtype == AP_F32 ? 4 : 2;
+    const bool bytes32 = !idx64 && (uint64_t)(array3_max_offset(vol_desc) + 1) * esz <= 0xFFFFFFFFull;
+    const size_t shm = ((cap * sizeof(float) + 15) & ~size_t(15)) + cap * (bytes32 ? 4 : 8);
     if (shm > 60 * 1024) return AP_ERR_UNSUPPORTED;
     SinoCfgC sc{V, nu, nv, s0, s1, s2};
     ColMap cm;
@@ -375,16 +675,18 @@ int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32
                 if constexpr (std::is_same<Tp, double>::value || std::is_same<Tp, bf16_t>::value) {
                     return ap_fail(AP_ERR_UNSUPPORTED, "sinogram dtype must be F32 or F16");
                 } else {
-                    if (idx64)
+                    if (!bytes32)
                         hipLaunchKernelGGL((k_project_columns<Tv, Tp, int64_t>), grid, dim3(kRowsF), shm, s, rc,
                                            sc, cm, mba, ssu, ssv, specs, (const Tv*)vol, vol_desc->strides[0],
                                            vol_desc->strides[1], vol_desc->strides[2], (Tp*)sino, (const Tp*)b,
                                            cap);
                     else
-                        hipLaunchKernelGGL((k_project_columns<Tv, Tp, int32_t>), grid, dim3(kRowsF), shm, s, rc,
+                        hipLaunchKernelGGL((k_project_columns<Tv, Tp, uint32_t>), grid, dim3(kRowsF), shm, s, rc,
                                            sc, cm, mba, ssu, ssv, specs, (const Tv*)vol,
-                                           (int32_t)vol_desc->strides[0], (int32_t)vol_desc->strides[1],
-                                           (int32_t)vol_desc->strides[2], (Tp*)sino, (const Tp*)b, cap);
+                                           (uint32_t)(vol_desc->strides[0] * sizeof(Tv)),
+                                           (uint32_t)(vol_desc->strides[1] * sizeof(Tv)),
+                                           (uint32_t)(vol_desc->strides[2] * sizeof(Tv)), (Tp*)sino, (const Tp*)b,
+                                           cap);
                     return ap_check_launch("project_columns");
                 }
             });
@@ -398,6 +700,8 @@ int32_t launch_backproject_columns(const RayCfg& rc, uint32_t V, uint32_t nu, ui
                                    int32_t mdt, const int64_t* ms, const void* mult, int32_t sino_dtype,
                                    const void* sino, bool idx64, hipStream_t s) {
     SinoCfgC sc{V, nu, nv, s0, s1, s2};
+    // the 64-bit fixed-point sums stay exact while views x rays per view < 2^33
+    if ((double)V * nu * nv * ssu * ssv >= 8589934592.0) return AP_ERR_UNSUPPORTED;
     const uint32_t nbx = (uint32_t)(rc.hi[0] - rc.lo[0] + kTX - 1) / kTX;
     const uint32_t nby = (uint32_t)(rc.hi[1] - rc.lo[1] + kTY - 1) / kTY;
     const uint32_t nbz = (uint32_t)(rc.hi[2] - rc.lo[2] + kTZ - 1) / kTZ;
