@@ -53,10 +53,11 @@ __device__ inline bool col_setup(const ViewRays& vr, const int lo[3], const int 
     const double dx = qx - vr.S[0], dy = qy - vr.S[1];
     const double l = sqrt(dx * dx + dy * dy);
     if (!(l > 0.0)) return false;
-    const double u[2] = {dx / l, dy / l};
+    const double il = 1.0 / l;
+    const double u[2] = {dx * il, dy * il};
     const double hc = -(vr.S[0] * u[0] + vr.S[1] * u[1]);
     C.z.hc = hc;
-    C.z.il = 1.0 / l;
+    C.z.il = il;
     C.z.sz = vr.S[2];
     C.z.qz0 = vr.Q00[2] + (double)ju * vr.U[2];
     C.z.vz = vr.V[2];
@@ -75,10 +76,11 @@ __device__ inline bool col_setup(const ViewRays& vr, const int lo[3], const int 
             continue;
         }
         const double P0 = floor(oab);
+        const double ru = 1.0 / u[a];
         C.P0[a] = (float)P0;
         C.sg[a] = u[a] > 0.0 ? 1 : -1;
-        C.r[a] = (float)(1.0 / u[a]);
-        C.c[a] = (float)((P0 - oab) / u[a]);
+        C.r[a] = (float)ru;
+        C.c[a] = (float)((P0 - oab) * ru);
         C.ifix[a] = 0;
     }
     return true;
@@ -186,7 +188,7 @@ __device__ inline bool lane_setup(const ColZ& C, float kv, int zlo, int zhi, Lan
     const double m = (qz - C.sz) * C.il;
     const double zab = C.sz + C.hc * m + 0.5 * (double)(zlo + zhi);
     const float mf = (float)m;
-    Z.sec = sqrtf(__builtin_fmaf(mf, mf, 1.f));
+    Z.sec = __builtin_amdgcn_sqrtf(__builtin_fmaf(mf, mf, 1.f));
     Z.u = mf;
     if (fabsf(mf) < 1e-20f) {
         Z.sg = 0;

@@ -68,6 +68,22 @@ __device__ __forceinline__ int first_above(const float* he, int n, float x) {
     return a;
 }
 
+// Quotient n / d for a wave-uniform divisor d >= 1 and 0 <= n < 2^22: a
+// float reciprocal estimate and one integer correction.
+struct UDiv {
+    float inv;
+    int d;
+};
+
+__device__ __forceinline__ UDiv make_udiv(int d) { return UDiv{1.f / (float)d, d}; }
+
+__device__ __forceinline__ int udiv(int n, const UDiv& D) {
+    int q = (int)((float)n * D.inv);
+    const int r = n - q * D.d;
+    q += (r >= D.d) - (r < 0);
+    return q;
+}
+
 // One row's z walk along segments [s0, s1) of a column list (he: segment
 // end parameters, ob: x-y offsets).  hp starts at the row's entry t0; every
 // z crossing before a segment's clamped end splits that segment.  Weights
@@ -330,7 +346,7 @@ struct WaveArea {
     ViewRays vr;                   // the current view (kept here, not in registers)
     ColEntry ce[kWCols];
     float he[kWCols][kListCap];
-    int ob[kWCols][kListCap];
+    int ob[kWCols][kListCap];      // LDS byte offsets of the cells' z lines
 };
 
 // pixel coordinates of the central projection (float), see project.hip
@@ -399,9 +415,10 @@ __device__ inline bool brick_footprint(const ViewRays& vr, const int blo[3], con
     return j0 <= j1 && k0 <= k1;
 }
 
-// One fixed-point term: w * wys rounded to an integer (|w * wys| < 2^30).
+// One fixed-point term: w * wys truncated to an integer (|w * wys| < 2^30;
+// the truncation bias is below one unit, about 1e-9 of the largest term).
 __device__ __forceinline__ unsigned long long fx_term(float w, float wys) {
-    return (unsigned long long)(long long)__float2int_rn(w * wys);
+    return (unsigned long long)(long long)(int)(w * wys);
 }
 
 template <typename Tv, typename Tp, typename Tm, typename IdxT>
@@ -431,8 +448,9 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
         if (!brick_footprint(make_view_rays(rc, specs, p), blo, bhi, cen, sc.nu, sc.nv, lane, j0, j1, k0, k1))
             continue;
         const int nk = k1 - k0 + 1, npix = (j1 - j0 + 1) * nk;
+        const UDiv dk = make_udiv(nk);
         for (int e = lane; e < npix; e += 64) {
-            const int jj = e / nk;
+            const int jj = udiv(e, dk);
             const float y = to_f32(sino[(int64_t)p * sc.s0 + (int64_t)(j0 + jj) * sc.s1 +
                                         (int64_t)(k0 + e - jj * nk) * sc.s2]);
             ymax = fmaxf(ymax, fabsf(y));   // a NaN sample is caught below
@@ -496,12 +514,12 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                         E.aux = fabsf(C.u[0]);
                         E.auy = fabsf(C.u[1]);
                         E.nseg = E.qx.n + E.qy.n + 1;
-                        W.ob[lane][0] = ((E.ix0 - blo[0]) * kTY + (E.iy0 - blo[1])) * kTZ;
+                        W.ob[lane][0] = ((E.ix0 - blo[0]) * kTY + (E.iy0 - blo[1])) * kTZ * 8;
                         // the last segment ends at h1; 3 zero-length padding
                         // segments repeat the final cell for the 4-wide walk
                         const int ixe = min(max(E.ix0 + E.qx.sg * E.qx.n, blo[0]), bhi[0] - 1);
                         const int iye = min(max(E.iy0 + E.qy.sg * E.qy.n, blo[1]), bhi[1] - 1);
-                        const int oe = ((ixe - blo[0]) * kTY + (iye - blo[1])) * kTZ;
+                        const int oe = ((ixe - blo[0]) * kTY + (iye - blo[1])) * kTZ * 8;
                         for (int q = 0; q < 4; q++) W.he[lane][E.nseg - 1 + q] = h1;
                         for (int q = 1; q < 4; q++) W.ob[lane][E.nseg - 1 + q] = oe;
                     }
@@ -518,7 +536,7 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                     const int ix = min(max(E.ix0 + E.qx.sg * (m + 1), blo[0]), bhi[0] - 1);
                     const int iy = min(max(E.iy0 + E.qy.sg * cy, blo[1]), bhi[1] - 1);
                     W.he[g][m + cy] = h;
-                    W.ob[g][m + cy + 1] = ((ix - blo[0]) * kTY + (iy - blo[1])) * kTZ;
+                    W.ob[g][m + cy + 1] = ((ix - blo[0]) * kTY + (iy - blo[1])) * kTZ * 8;
                 }
                 if (m < E.qy.n) {
                     const float h = seq_at(E.qy, m);
@@ -526,29 +544,31 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                     const int ix = min(max(E.ix0 + E.qx.sg * cx, blo[0]), bhi[0] - 1);
                     const int iy = min(max(E.iy0 + E.qy.sg * (m + 1), blo[1]), bhi[1] - 1);
                     W.he[g][m + cx] = h;
-                    W.ob[g][m + cx + 1] = ((ix - blo[0]) * kTY + (iy - blo[1])) * kTZ;
+                    W.ob[g][m + cx + 1] = ((ix - blo[0]) * kTY + (iy - blo[1])) * kTZ * 8;
                 }
             }
             wave_sync_lds();
             // (sub-column, sub-row) pairs: every lane scatters one ray at a time;
             // the sinogram value of the lane's next pair is fetched one pair ahead
             const int npair = nc * nrow;
+            const UDiv drow = make_udiv(nrow), dsv = make_udiv((int)ssv);
             auto yval = [&](int e) -> float {
                 if (e >= npair) return 0.f;
-                const int g = e / nrow;
-                const int k = k0 + (e - g * nrow) / (int)ssv;
+                const int g = udiv(e, drow);
+                const int k = k0 + udiv(e - g * nrow, dsv);
                 return to_f32(sino[(int64_t)p * sc.s0 + (int64_t)W.ce[g].j * sc.s1 + (int64_t)k * sc.s2]);
             };
             float y_next = yval(lane);
             for (int e = lane; e < npair; e += 64) {
                 const float y = y_next;
                 y_next = yval(e + 64);
-                const int g = e / nrow, kk = e - g * nrow;
+                const int g = udiv(e, drow), kk = e - g * nrow;
                 const ColEntry& E = W.ce[g];
                 if (E.nseg == 0 || y == 0.f) continue;
-                const int k = k0 + kk / (int)ssv;
+                const int kq = udiv(kk, dsv);
+                const int k = k0 + kq;
                 LaneZ Z;
-                if (!lane_setup(E.z, (float)k + sub_off((uint32_t)(kk % (int)ssv), ssv), rc.lo[2], rc.hi[2], Z))
+                if (!lane_setup(E.z, (float)k + sub_off((uint32_t)(kk - kq * (int)ssv), ssv), rc.lo[2], rc.hi[2], Z))
                     continue;
                 float zin, zout;
                 lane_box(Z, blo[2], bhi[2], zin, zout);
@@ -567,18 +587,27 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                 float hp = t0;
                 if (Z.u > -kSingleSlope && Z.u < kSingleSlope) {
                     // at most one z crossing per segment: 4 segments per step
-                    // (LDS byte offsets), ending with the chunk that reaches t1;
-                    // the add after a crossing is skipped where there is none
+                    // (LDS byte offsets), ending with the chunk that reaches t1
                     char* accb = reinterpret_cast<char*>(acc);
                     int zb = zl * 8;
                     const int dzb = Z.sg * 8;
-                    for (int s = s0 & ~3;; s += 4) {
-                        const float4 h4 = *reinterpret_cast<const float4*>(he + s);
-                        const int4 o4 = *reinterpret_cast<const int4*>(ob + s);
+                    // the list of chunk i+1 is read before the adds of chunk i
+                    // issue, so its wait does not include them (LDS returns in
+                    // order); chunks start at most at kListCap - 8, so the read
+                    // ahead stays inside the list; the padding ends the walk
+                    // with the chunk whose last entry reaches t1
+                    int s = s0 & ~3;
+                    float4 h4 = *reinterpret_cast<const float4*>(he + s);
+                    int4 o4 = *reinterpret_cast<const int4*>(ob + s);
+                    // complete the first read here, so that the loop head
+                    // never waits for outstanding adds
+                    asm volatile("" : "+v"(h4.x), "+v"(h4.y), "+v"(h4.z), "+v"(h4.w), "+v"(o4.x), "+v"(o4.y),
+                                 "+v"(o4.z), "+v"(o4.w));
+                    for (;;) {
+                        const float4 h4n = *reinterpret_cast<const float4*>(he + s + 4);
+                        const int4 o4n = *reinterpret_cast<const int4*>(ob + s + 4);
                         const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
-                        const int ov[4] = {o4.x * 8, o4.y * 8, o4.z * 8, o4.w * 8};
-                        // weights and addresses first, then the adds: no LDS
-                        // read is outstanding while the atomics issue
+                        const int ov[4] = {o4.x, o4.y, o4.z, o4.w};
                         float wb[4], wa[4];
                         int a0[4], a1[4];
 #pragma unroll
@@ -599,18 +628,23 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                         // (which waits on the list read) between the adds
                         asm volatile("" : "+v"(a0[0]), "+v"(a0[1]), "+v"(a0[2]), "+v"(a0[3]), "+v"(a1[0]),
                                      "+v"(a1[1]), "+v"(a1[2]), "+v"(a1[3]));
+                        // all 8 adds issue unconditionally (zero terms add nothing):
+                        // a fixed count of LDS operations after the read ahead
+                        // lets its wait skip them
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
                             atomicAdd(reinterpret_cast<unsigned long long*>(accb + a0[q]), fx_term(wb[q], wys));
-                            if (wa[q] != 0.f)
-                                atomicAdd(reinterpret_cast<unsigned long long*>(accb + a1[q]), fx_term(wa[q], wys));
+                            atomicAdd(reinterpret_cast<unsigned long long*>(accb + a1[q]), fx_term(wa[q], wys));
                         }
-                        if (h4.w >= t1 || s + 4 >= E.nseg) break;
+                        if (h4.w >= t1) break;
+                        s += 4;
+                        h4 = h4n;
+                        o4 = o4n;
                     }
                 } else {
                     for (int s = s0; s < E.nseg; s++) {
                         const float cend = __builtin_amdgcn_fmed3f(he[s], t0, t1);
-                        const int o = ob[s];
+                        const int o = ob[s] / 8;
                         while (hz < cend) {
                             atomicAdd(&acc[o + zl], fx_term(hz - hp, wys));
                             hp = hz;
