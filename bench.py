@@ -12,6 +12,13 @@ iteration on the 256^3 x 360 x 384^2 workload, and the FP32 OpenMP CPU port of
 the reference semantics timed on a bounded sample of the same workload.
 
     python bench.py [--steps K] [--warmup W] [--volume-dtype f32|bf16]
+    python bench.py --calibrate-copy     # plain 1 GiB device copies, for
+                                         # scaling rocprofv3 FETCH/WRITE_SIZE
+
+roofline.traffic (HBM bytes per launch of the dominant kernel) is read from
+profiles/hbm_traffic_siddon_512cube_720views.txt, the summary of two
+rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters
+read for --calibrate-copy; it is null when that file is absent.
 """
 
 from __future__ import annotations
@@ -25,6 +32,9 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "GUPS fwd+adjoint proj at 512\u00b3\u00d7720 views"
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "hbm_traffic_siddon_512cube_720views.txt")
 
 
 def log(msg: str) -> None:
@@ -119,16 +129,49 @@ def cpu_baseline(vol_np, pg, n, w, views: int) -> dict:
                       f"voxel-driven backprojector (fwd {t_fwd:.2f} s, bp {t_bp:.2f} s)"}
 
 
+def read_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (a line 'traffic_bytes_per_launch <kernel> <fetch> <write>')."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2]) + float(parts[3])
+    except OSError:
+        return None
+    return None
+
+
+def calibrate_copy() -> int:
+    """Two plain device-to-device copies of 1 GiB each (2 GiB read, 2 GiB
+    written), the reference for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    n = 256 * 1024 * 1024
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each": n * 4, "copies": 2}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--volume-dtype", choices=["f32", "bf16"], default="f32")
-    ap.add_argument("--cpu-views", type=int, default=4)
+    ap.add_argument("--cpu-views", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sirt", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the projector shards by views across GPUs, which this build does not run)", file=sys.stderr)
@@ -183,10 +226,12 @@ def main() -> int:
     s_v = 2 if args.volume_dtype == "bf16" else 4
     b_op = V * (n**3 * s_v + nu * nv * 4)
     dom = "adjoint" if t_at >= t_a else "forward"
+    dom_kernel = "k_backproject_columns" if dom == "adjoint" else "k_project_columns"
     t_dom = max(t_a, t_at)
     achieved = b_op / t_dom / 1e9
+    traffic = read_traffic(dom_kernel) if args.volume_dtype == "f32" else None
     out = {
-        "metric": "GUPS fwd+adjoint proj at 512^3x720 views",
+        "metric": METRIC,
         "value": gups,
         "unit": "GUPS",
         "n_gpus": 1,
@@ -206,9 +251,10 @@ def main() -> int:
         "gups_forward": voxel_views / t_a / 1e9,
         "gups_adjoint": voxel_views / t_at / 1e9,
         "dot_test_rel": dot_rel,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "model": "B_op = V (N^3 s_v + nu nv s_p) bytes per operator application"},
+        "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "model": "B_op = V (N^3 s_v + nu nv s_p) bytes per operator application; time = the "
+                              "operator call on its stream (HIP events)"},
     }
     if not args.no_sirt:
         log("SIRT 256^3 x 360 x 384^2")
