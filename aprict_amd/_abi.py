@@ -157,7 +157,16 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ap_array_dot": (_I32, [_A3, _P, _A3, _P, _P, _P]),
     "ap_tv_gradient": (_I32, [_A3, _P, _A3, _P, _F32, _F32, _P]),
     "ap_sirt_iterate": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _P, _P]),
+    "ap_tune_count": (_I32, [_I32]),
+    "ap_tune_get": (_I32, [_I32]),
+    "ap_tune_set": (_I32, [_I32, _I32]),
 }
+
+# launch-shape tuning ops (include/aprict_hip.h AP_TUNE_*)
+TUNE_PROJECT_RAY = 0
+TUNE_PROJECT_COLUMN = 1
+TUNE_BACKPROJECT = 2
+TUNE_NUM_OPS = 3
 
 
 class AprictError(RuntimeError):

@@ -1,5 +1,6 @@
 // abi.hip -- error reporting, struct-layout checks and descriptor validation
 // of the C ABI declared in include/aprict_hip.h.
+#include <atomic>
 #include <stdarg.h>
 #include <stddef.h>
 #include <stdio.h>
@@ -58,7 +59,28 @@ int32_t ap_check_array3(const ap_array3* d, const void* ptr, const char* name,
     return AP_OK;
 }
 
+namespace {
+constexpr int32_t kTuneCount[AP_TUNE_NUM_OPS] = {3, 3, 3};
+std::atomic<int32_t> g_tune[AP_TUNE_NUM_OPS] = {};
+}  // namespace
+
+int32_t ap_tune_variant(int32_t op) {
+    return (op >= 0 && op < AP_TUNE_NUM_OPS) ? g_tune[op].load(std::memory_order_relaxed) : 0;
+}
+
 extern "C" {
+
+int32_t ap_tune_count(int32_t op) { return (op >= 0 && op < AP_TUNE_NUM_OPS) ? kTuneCount[op] : -1; }
+
+int32_t ap_tune_get(int32_t op) { return (op >= 0 && op < AP_TUNE_NUM_OPS) ? ap_tune_variant(op) : -1; }
+
+int32_t ap_tune_set(int32_t op, int32_t variant) {
+    if (op < 0 || op >= AP_TUNE_NUM_OPS) return ap_fail(AP_ERR_INVALID, "unknown tuning op %d", op);
+    if (variant < 0 || variant >= kTuneCount[op])
+        return ap_fail(AP_ERR_INVALID, "tuning op %d has %d variants, got %d", op, kTuneCount[op], variant);
+    g_tune[op].store(variant, std::memory_order_relaxed);
+    return AP_OK;
+}
 
 const char* ap_last_error(void) { return g_last_error; }
 

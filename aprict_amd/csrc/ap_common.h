@@ -10,6 +10,8 @@
 // error reporting (ap_last_error is thread-local)
 // ---------------------------------------------------------------------------
 int32_t ap_fail(int32_t code, const char* fmt, ...);
+// selected launch-shape variant of a tunable operation (AP_TUNE_*)
+int32_t ap_tune_variant(int32_t op);
 int32_t ap_check_launch(const char* what);
 
 #define AP_REQUIRE(cond, ...)                                   \

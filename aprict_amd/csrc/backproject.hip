@@ -11,8 +11,12 @@
 
 namespace {
 
-constexpr int kBz = 64, kBy = 4;
+constexpr int kBz = 64;
+// voxel rows (along y) per workgroup: AP_TUNE_BACKPROJECT variants; every
+// workgroup has at least kViewBatch threads to stage the view maps
+constexpr int kBpRowsY[3] = {4, 2, 1};
 constexpr int kViewBatch = 64;
+static_assert(kBz * kBpRowsY[2] >= kViewBatch, "every variant stages a full view batch");
 
 // Per-view constants (FP32) of the central projection x -> detector:
 //   cu = (Au.x + au)/(W.x + w0),  cv = (Av.x + av)/(W.x + w0)  (pixels from
@@ -200,7 +204,7 @@ __device__ float bp_one_view(const BpCfg& c, const ViewMap& m, const Tp* __restr
     return val;
 }
 
-template <typename Tv, typename Tp, typename Tm>
+template <int kBy, typename Tv, typename Tp, typename Tm>
 __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
     BpCfg c, const ap_proj_spec* __restrict__ specs, uint32_t num_projs,
     Tv* __restrict__ vol, int64_t vs0, int64_t vs1, int64_t vs2,
@@ -294,8 +298,10 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
     c.nu = sino_desc->shape[1];
     c.nv = sino_desc->shape[2];
     c.s0 = sino_desc->strides[0]; c.s1 = sino_desc->strides[1]; c.s2 = sino_desc->strides[2];
-    dim3 block(kBz, kBy, 1);
-    dim3 grid((c.shp[2] + kBz - 1) / kBz, (c.shp[1] + kBy - 1) / kBy, c.shp[0]);
+    const int var = ap_tune_variant(AP_TUNE_BACKPROJECT);
+    const uint32_t rows_y = (uint32_t)kBpRowsY[var];
+    dim3 block(kBz, rows_y, 1);
+    dim3 grid((c.shp[2] + kBz - 1) / kBz, (c.shp[1] + rows_y - 1) / rows_y, c.shp[0]);
     const int32_t mdt = multiplier ? mult_desc->dtype : vol_desc->dtype;
     const int64_t* ms = multiplier ? mult_desc->strides : vol_desc->strides;
     return dispatch_dtype(vol_desc->dtype, [&](auto tv) -> int32_t {
@@ -308,10 +314,14 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
                               std::is_same<Tm, double>::value || std::is_same<Tp, bf16_t>::value) {
                     return ap_fail(AP_ERR_UNSUPPORTED, "unsupported dtype combination");
                 } else {
-                    hipLaunchKernelGGL((k_backproject_voxel<Tv, Tp, Tm>), grid, block, 0, s, c, specs,
-                                       num_projs, (Tv*)vol, vol_desc->strides[0], vol_desc->strides[1],
-                                       vol_desc->strides[2], (const Tm*)multiplier, ms[0], ms[1], ms[2],
-                                       (const Tp*)sino);
+                    auto launch = [&](auto kern) {
+                        hipLaunchKernelGGL(kern, grid, block, 0, s, c, specs, num_projs, (Tv*)vol,
+                                           vol_desc->strides[0], vol_desc->strides[1], vol_desc->strides[2],
+                                           (const Tm*)multiplier, ms[0], ms[1], ms[2], (const Tp*)sino);
+                    };
+                    if (var == 0) launch(k_backproject_voxel<kBpRowsY[0], Tv, Tp, Tm>);
+                    if (var == 1) launch(k_backproject_voxel<kBpRowsY[1], Tv, Tp, Tm>);
+                    if (var == 2) launch(k_backproject_voxel<kBpRowsY[2], Tv, Tp, Tm>);
                     return ap_check_launch("backproject_voxel");
                 }
             });

@@ -20,7 +20,9 @@ namespace {
 
 constexpr int kXcdC = 8;
 constexpr int kViewGroupC = 8;
-constexpr int kRowsF = 256;        // rows per forward workgroup (4 waves)
+// detector rows per forward workgroup (AP_TUNE_PROJECT_COLUMN variants);
+// waves always hold 64 consecutive rows, so every variant gives identical sums
+constexpr int kColRows[3] = {256, 128, 64};
 
 struct SinoCfgC {
     uint32_t V, nu, nv;
@@ -236,7 +238,7 @@ __device__ inline RowWalk row_setup(const Col2D& C, float h0, float h1, float kv
     return W;
 }
 
-template <typename Tv, typename Tp, typename OffT>
+template <int kRowsF, typename Tv, typename Tp, typename OffT>
 __global__ __launch_bounds__(kRowsF) void k_project_columns(
     RayCfg rc, SinoCfgC sc, ColMap cm, float mba, uint32_t ssu, uint32_t ssv,
     const ap_proj_spec* __restrict__ specs, const Tv* __restrict__ vol, OffT vsx, OffT vsy, OffT vsz,
@@ -421,6 +423,25 @@ __device__ __forceinline__ unsigned long long fx_term(float w, float wys) {
     return (unsigned long long)(long long)(int)(w * wys);
 }
 
+// All terms of one ray share the sign of wys, so the high word of the 64-bit
+// two's-complement term is a per-ray constant: 0 when wys >= 0, all ones
+// otherwise.  A ray with wys < 0 adds trunc(w * wys - 1) <= -1, which keeps
+// the constant high word exact (zero weights included); the extra unit is
+// within the quantisation (about 2e-9 of the largest term).
+struct FxRay {
+    float wys, bias;
+    uint32_t hi;
+};
+
+__device__ __forceinline__ FxRay fx_ray(float wys) {
+    return FxRay{wys, wys < 0.f ? -1.f : 0.f, wys < 0.f ? 0xFFFFFFFFu : 0u};
+}
+
+__device__ __forceinline__ unsigned long long fx_term(float w, const FxRay& R) {
+    const uint32_t lo = (uint32_t)(int)__builtin_fmaf(w, R.wys, R.bias);
+    return ((unsigned long long)R.hi << 32) | lo;
+}
+
 template <typename Tv, typename Tp, typename Tm, typename IdxT>
 __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
     RayCfg rc, SinoCfgC sc, float scale, uint32_t ssu, uint32_t ssv,
@@ -596,6 +617,7 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                     // order); chunks start at most at kListCap - 8, so the read
                     // ahead stays inside the list; the padding ends the walk
                     // with the chunk whose last entry reaches t1
+                    const FxRay R = fx_ray(wys);
                     int s = s0 & ~3;
                     float4 h4 = *reinterpret_cast<const float4*>(he + s);
                     int4 o4 = *reinterpret_cast<const int4*>(ob + s);
@@ -603,11 +625,13 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                     // never waits for outstanding adds
                     asm volatile("" : "+v"(h4.x), "+v"(h4.y), "+v"(h4.z), "+v"(h4.w), "+v"(o4.x), "+v"(o4.y),
                                  "+v"(o4.z), "+v"(o4.w));
-                    for (;;) {
-                        const float4 h4n = *reinterpret_cast<const float4*>(he + s + 4);
-                        const int4 o4n = *reinterpret_cast<const int4*>(ob + s + 4);
-                        const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
-                        const int ov[4] = {o4.x, o4.y, o4.z, o4.w};
+                    // one chunk of 4 segments: weights and addresses first, then
+                    // the adds, so no LDS read is outstanding while they issue;
+                    // all 8 adds issue (zero terms add nothing), a fixed count of
+                    // LDS operations after the read ahead
+                    auto chunk = [&](const float4& hc4, const int4& oc4) {
+                        const float hv[4] = {hc4.x, hc4.y, hc4.z, hc4.w};
+                        const int ov[4] = {oc4.x, oc4.y, oc4.z, oc4.w};
                         float wb[4], wa[4];
                         int a0[4], a1[4];
 #pragma unroll
@@ -628,18 +652,24 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                         // (which waits on the list read) between the adds
                         asm volatile("" : "+v"(a0[0]), "+v"(a0[1]), "+v"(a0[2]), "+v"(a0[3]), "+v"(a1[0]),
                                      "+v"(a1[1]), "+v"(a1[2]), "+v"(a1[3]));
-                        // all 8 adds issue unconditionally (zero terms add nothing):
-                        // a fixed count of LDS operations after the read ahead
-                        // lets its wait skip them
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
-                            atomicAdd(reinterpret_cast<unsigned long long*>(accb + a0[q]), fx_term(wb[q], wys));
-                            atomicAdd(reinterpret_cast<unsigned long long*>(accb + a1[q]), fx_term(wa[q], wys));
+                            atomicAdd(reinterpret_cast<unsigned long long*>(accb + a0[q]), fx_term(wb[q], R));
+                            atomicAdd(reinterpret_cast<unsigned long long*>(accb + a1[q]), fx_term(wa[q], R));
                         }
+                    };
+                    // two chunks per turn with alternating list registers
+                    for (;;) {
+                        const float4 h4b = *reinterpret_cast<const float4*>(he + s + 4);
+                        const int4 o4b = *reinterpret_cast<const int4*>(ob + s + 4);
+                        chunk(h4, o4);
                         if (h4.w >= t1) break;
                         s += 4;
-                        h4 = h4n;
-                        o4 = o4n;
+                        h4 = *reinterpret_cast<const float4*>(he + s + 4);
+                        o4 = *reinterpret_cast<const int4*>(ob + s + 4);
+                        chunk(h4b, o4b);
+                        if (h4b.w >= t1) break;
+                        s += 4;
                     }
                 } else {
                     for (int s = s0; s < E.nseg; s++) {
@@ -691,8 +721,10 @@ int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32
     const size_t shm = ((cap * sizeof(float) + 15) & ~size_t(15)) + cap * (bytes32 ? 4 : 8);
     if (shm > 60 * 1024) return AP_ERR_UNSUPPORTED;
     SinoCfgC sc{V, nu, nv, s0, s1, s2};
+    const int var = ap_tune_variant(AP_TUNE_PROJECT_COLUMN);
+    const uint32_t rows = (uint32_t)kColRows[var];
     ColMap cm;
-    cm.nkb = (nv + kRowsF - 1) / kRowsF;
+    cm.nkb = (nv + rows - 1) / rows;
     cm.nu = nu;
     const uint64_t total = (uint64_t)((V + kViewGroupC - 1) / kViewGroupC) * kViewGroupC * nu * cm.nkb;
     if (total >= (1ull << 31)) return AP_ERR_UNSUPPORTED;
@@ -709,18 +741,24 @@ int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32
                 if constexpr (std::is_same<Tp, double>::value || std::is_same<Tp, bf16_t>::value) {
                     return ap_fail(AP_ERR_UNSUPPORTED, "sinogram dtype must be F32 or F16");
                 } else {
-                    if (!bytes32)
-                        hipLaunchKernelGGL((k_project_columns<Tv, Tp, int64_t>), grid, dim3(kRowsF), shm, s, rc,
-                                           sc, cm, mba, ssu, ssv, specs, (const Tv*)vol, vol_desc->strides[0],
-                                           vol_desc->strides[1], vol_desc->strides[2], (Tp*)sino, (const Tp*)b,
-                                           cap);
-                    else
-                        hipLaunchKernelGGL((k_project_columns<Tv, Tp, uint32_t>), grid, dim3(kRowsF), shm, s, rc,
-                                           sc, cm, mba, ssu, ssv, specs, (const Tv*)vol,
-                                           (uint32_t)(vol_desc->strides[0] * sizeof(Tv)),
-                                           (uint32_t)(vol_desc->strides[1] * sizeof(Tv)),
-                                           (uint32_t)(vol_desc->strides[2] * sizeof(Tv)), (Tp*)sino, (const Tp*)b,
-                                           cap);
+                    auto launch = [&](auto kern, auto off, uint32_t esz_scale) {
+                        using OffT = decltype(off);
+                        hipLaunchKernelGGL(kern, grid, dim3(rows), shm, s, rc, sc, cm, mba, ssu, ssv, specs,
+                                           (const Tv*)vol, (OffT)(vol_desc->strides[0] * esz_scale),
+                                           (OffT)(vol_desc->strides[1] * esz_scale),
+                                           (OffT)(vol_desc->strides[2] * esz_scale), (Tp*)sino, (const Tp*)b, cap);
+                    };
+                    // 64-bit element offsets, or 32-bit byte offsets
+                    if (!bytes32) {
+                        if (var == 0) launch(k_project_columns<kColRows[0], Tv, Tp, int64_t>, int64_t{}, 1);
+                        if (var == 1) launch(k_project_columns<kColRows[1], Tv, Tp, int64_t>, int64_t{}, 1);
+                        if (var == 2) launch(k_project_columns<kColRows[2], Tv, Tp, int64_t>, int64_t{}, 1);
+                    } else {
+                        const uint32_t e = (uint32_t)sizeof(Tv);
+                        if (var == 0) launch(k_project_columns<kColRows[0], Tv, Tp, uint32_t>, uint32_t{}, e);
+                        if (var == 1) launch(k_project_columns<kColRows[1], Tv, Tp, uint32_t>, uint32_t{}, e);
+                        if (var == 2) launch(k_project_columns<kColRows[2], Tv, Tp, uint32_t>, uint32_t{}, e);
+                    }
                     return ap_check_launch("project_columns");
                 }
             });

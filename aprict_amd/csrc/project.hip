@@ -22,8 +22,10 @@
 namespace {
 
 constexpr int kWaveV = 64;   // pixels along v per wave
-constexpr int kRowsU = 4;    // waves (rows along u) per block
-constexpr int kBatch = 4;    // voxels walked per batch of gathers
+// Launch-shape variants (AP_TUNE_PROJECT_RAY): waves (pixel rows along u)
+// per block and voxels walked per batch of gathers.
+constexpr int kRayRowsU[3] = {4, 2, 1};
+constexpr int kRayBatch[3] = {4, 4, 8};
 constexpr int kViewGroup = 8;
 constexpr int kXcd = 8;      // MI355X: 8 XCDs, workgroups dispatched round-robin
 
@@ -57,7 +59,7 @@ __device__ __forceinline__ float sub_offset(uint32_t s, uint32_t ss) {
     return -0.5f + ((float)s + 0.5f) / (float)ss;
 }
 
-template <typename Tv, typename Tp, typename IdxT>
+template <int kRowsU, int kBatch, typename Tv, typename Tp, typename IdxT>
 __global__ __launch_bounds__(kWaveV* kRowsU) void k_project_voxel(
     RayCfg rc, SinoCfg sc, TileMap tm, float mba, uint32_t ssu, uint32_t ssv,
     const ap_proj_spec* __restrict__ specs, const Tv* __restrict__ vol, IdxT vsx, IdxT vsy, IdxT vsz,
@@ -323,15 +325,17 @@ extern "C" int32_t ap_project_voxel(const ap_config_cb_project* cfg, const ap_pr
     if (num_projs == 0 || sc.nu == 0 || sc.nv == 0) return AP_OK;
     const int64_t maxoff = array3_max_offset(vol_desc);
     const bool idx64 = maxoff >= (int64_t)INT32_MAX;
+    const int var = ap_tune_variant(AP_TUNE_PROJECT_RAY);
+    const uint32_t rows_u = (uint32_t)kRayRowsU[var];
     TileMap tm;
     tm.nk = (sc.nv + kWaveV - 1) / kWaveV;
-    tm.nj = (sc.nu + kRowsU - 1) / kRowsU;
+    tm.nj = (sc.nu + rows_u - 1) / rows_u;
     const uint64_t groups = (num_projs + kViewGroup - 1) / kViewGroup;
     const uint64_t total = groups * kViewGroup * tm.nj * tm.nk;
     AP_REQUIRE(total < (1ull << 31), "too many tiles in one call");
     tm.total = (uint32_t)total;
     tm.per_xcd = (tm.total + kXcd - 1) / kXcd;
-    dim3 block(kWaveV, kRowsU, 1);
+    dim3 block(kWaveV, rows_u, 1);
     dim3 grid(tm.per_xcd * kXcd, 1, 1);
     hipStream_t s = (hipStream_t)stream;
     const float mba = cfg->multiply_before_adding;
@@ -351,16 +355,20 @@ extern "C" int32_t ap_project_voxel(const ap_config_cb_project* cfg, const ap_pr
                 if constexpr (std::is_same<Tp, double>::value || std::is_same<Tp, bf16_t>::value) {
                     return ap_fail(AP_ERR_UNSUPPORTED, "sinogram dtype must be F32 or F16");
                 } else {
+                    auto launch = [&](auto kern, auto idx) {
+                        using IdxT = decltype(idx);
+                        hipLaunchKernelGGL(kern, grid, block, 0, s, rc, sc, tm, mba, cfg->ssu, cfg->ssv, specs,
+                                           (const Tv*)vol, (IdxT)vol_desc->strides[0], (IdxT)vol_desc->strides[1],
+                                           (IdxT)vol_desc->strides[2], (Tp*)sino, (const Tp*)b, skip);
+                    };
                     if (idx64) {
-                        hipLaunchKernelGGL((k_project_voxel<Tv, Tp, int64_t>), grid, block, 0, s, rc, sc, tm,
-                                           mba, cfg->ssu, cfg->ssv, specs, (const Tv*)vol,
-                                           vol_desc->strides[0], vol_desc->strides[1],
-                                           vol_desc->strides[2], (Tp*)sino, (const Tp*)b, skip);
+                        if (var == 0) launch(k_project_voxel<kRayRowsU[0], kRayBatch[0], Tv, Tp, int64_t>, int64_t{});
+                        if (var == 1) launch(k_project_voxel<kRayRowsU[1], kRayBatch[1], Tv, Tp, int64_t>, int64_t{});
+                        if (var == 2) launch(k_project_voxel<kRayRowsU[2], kRayBatch[2], Tv, Tp, int64_t>, int64_t{});
                     } else {
-                        hipLaunchKernelGGL((k_project_voxel<Tv, Tp, int32_t>), grid, block, 0, s, rc, sc, tm,
-                                           mba, cfg->ssu, cfg->ssv, specs, (const Tv*)vol,
-                                           (int32_t)vol_desc->strides[0], (int32_t)vol_desc->strides[1],
-                                           (int32_t)vol_desc->strides[2], (Tp*)sino, (const Tp*)b, skip);
+                        if (var == 0) launch(k_project_voxel<kRayRowsU[0], kRayBatch[0], Tv, Tp, int32_t>, int32_t{});
+                        if (var == 1) launch(k_project_voxel<kRayRowsU[1], kRayBatch[1], Tv, Tp, int32_t>, int32_t{});
+                        if (var == 2) launch(k_project_voxel<kRayRowsU[2], kRayBatch[2], Tv, Tp, int32_t>, int32_t{});
                     }
                     return ap_check_launch("project_voxel");
                 }

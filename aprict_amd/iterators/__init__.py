@@ -1,6 +1,7 @@
 """Iterative reconstructors (``aprict.iterators`` counterpart)."""
 
+from .batched import BatchedSIRT
 from .landweber_vram import Landweber_flatpanel
 from .sirt_vram import SIRT_flatpanel
 
-__all__ = ["Landweber_flatpanel", "SIRT_flatpanel"]
+__all__ = ["BatchedSIRT", "Landweber_flatpanel", "SIRT_flatpanel"]

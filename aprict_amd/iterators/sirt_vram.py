@@ -75,6 +75,9 @@ class SIRT_flatpanel:  # noqa: N801
             M.soften_to_xy_cylinder(config.soft_from_cylinder_relrad)
         self._preconditioner_volume = M
         self._proj_scratch_space = p
+        # upload the view table now: iterate() then makes no host->device copy
+        # and can be recorded into a HIP graph
+        projs.specs_device()
         torch.cuda.synchronize(volume.device)
 
     @property

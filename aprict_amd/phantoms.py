@@ -86,7 +86,88 @@ CONFIGS = {
     3: {"n_primitives": 40, "scale": 30.0, "views": 720, "nu": 512, "nv": 512, "pitch": 0.5, "seed": 3},
     4: {"N": 512, "voxwid": 0.48, "n_ellipsoids": 30, "views": 720, "nu": 768, "nv": 768, "pitch": 1.0,
         "seed": 4},
+    5: {"problems": 64, "seed": 5, "sirt_iters": 10, "storage": "float16"},
 }
+
+
+def _rot(axis: np.ndarray, ang: float) -> np.ndarray:
+    """Rotation matrix about the unit vector ``axis`` by ``ang`` radians (Rodrigues)."""
+    a = axis / np.linalg.norm(axis)
+    k = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(ang) * k + (1 - np.cos(ang)) * (k @ k)
+
+
+def tilted_geometry(num_views: int, n: int, pitch: float, *, inplane_deg: float, outplane_deg: float,
+                    offset_uv: tuple[float, float], R: float = 500.0, L: float = 1000.0) -> cbfp.ProjGeom:  # noqa: N803
+    """Circular scan whose square ``n x n`` panel is tilted in its plane (about
+    the detector normal) and out of it (about the u axis), and shifted by
+    ``offset_uv`` mm along u and v; u and v stay orthogonal."""
+    base = circular_geometry(num_views, n, n, pitch, R=R, L=L)
+    src = base.source_pos.astype(np.float64)
+    det = base.det_pos.astype(np.float64)
+    u0 = base.det_uvec.astype(np.float64)
+    v0 = np.broadcast_to(base.det_vvec.astype(np.float64), u0.shape)
+    us, vs, ds = [], [], []
+    for i in range(num_views):
+        nrm = np.cross(u0[i], v0[i])
+        r = _rot(u0[i], np.deg2rad(outplane_deg)) @ _rot(nrm, np.deg2rad(inplane_deg))
+        u, v = r @ u0[i], r @ v0[i]
+        us.append(u)
+        vs.append(v)
+        ds.append(det[i] + offset_uv[0] * u / np.linalg.norm(u) + offset_uv[1] * v / np.linalg.norm(v))
+    return cbfp.ProjGeom(source_pos=src, det_pos=np.array(ds), det_uvec=np.array(us), det_vvec=np.array(vs),
+                         nu=n, nv=n)
+
+
+def batched_problems(count: int = 64, seed: int = 5, *, n_range=(32, 192), views_range=(60, 360),
+                     det_range=(64, 256), pitch: float = 1.0, tilt_deg: float = 5.0, offset_mm: float = 10.0,
+                     n_ellipsoids: int = 10, R: float = 500.0, L: float = 1000.0) -> list[dict]:  # noqa: N803
+    """Uneven reconstruction problems: volume side N, view count V and square
+    detector side n drawn uniformly; detector tilts U[-tilt, tilt] degrees in
+    and out of its plane and centre offsets U[-offset, offset] mm along u and v;
+    the voxel width is fitted so that the volume's circumscribed cylinder and
+    its height project inside the detector.  Each entry holds the geometry,
+    the FORBILD ellipsoid phantom and the sizes."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n_vol = int(rng.integers(n_range[0], n_range[1] + 1))
+        views = int(rng.integers(views_range[0], views_range[1] + 1))
+        n_det = int(rng.integers(det_range[0], det_range[1] + 1))
+        tilt_in, tilt_out = rng.uniform(-tilt_deg, tilt_deg, 2)
+        off = rng.uniform(-offset_mm, offset_mm, 2)
+        # usable half-width at the detector, back-projected to the isocentre,
+        # with room for the tilt and for magnification of the near side
+        half_det = n_det * pitch / 2 - np.abs(off).max() - n_det * pitch * np.sin(np.deg2rad(tilt_deg))
+        half_iso = 0.85 * half_det * R / L
+        # circumscribed cylinder radius r = N w / sqrt(2); near-side magnification L / (R - r)
+        w = half_iso * np.sqrt(2) / n_vol
+        w = w / (1 + n_vol * w / np.sqrt(2) / R)
+        pg = tilted_geometry(views, n_det, pitch, inplane_deg=float(tilt_in), outplane_deg=float(tilt_out),
+                             offset_uv=(float(off[0]), float(off[1])), R=R, L=L)
+        shapes = random_ellipsoids(n_ellipsoids, n_vol * w / 2, seed * 1000 + i)
+        out.append({"geometry": pg, "shapes": shapes, "N": n_vol, "voxwid": float(w), "views": views,
+                    "det": n_det, "tilt_deg": (float(tilt_in), float(tilt_out)), "offset_mm": tuple(off.tolist())})
+    return out
+
+
+def batched_problem_data(entry: dict, storage: str = "float16", device=None):
+    """GPU data of one ``batched_problems`` entry: a zero initial volume and
+    the measurements ``b = A x_true`` (x_true: the entry's ellipsoids point-
+    sampled at voxel centres; A computed in FP32, then stored as ``storage``)."""
+    import torch  # noqa: PLC0415
+
+    from .volume import FBVolume, Volume  # noqa: PLC0415
+
+    n, w = entry["N"], entry["voxwid"]
+    truth = Volume.zeros((n, n, n), voxwid=w, dtype=np.float32, device=device)
+    FBVolume(entry["shapes"]).voxelise(truth)
+    ps = cbfp.ProjectionSet(entry["geometry"], device=device)
+    cbfp.project(cbfp.ConfigProject(), truth, ps)
+    tdt = {"float16": torch.float16, "float32": torch.float32}[storage]
+    meas = cbfp.ProjectionSet(entry["geometry"], data=ps.data.to(tdt), device=device)
+    x0 = Volume.zeros((n, n, n), voxwid=w, dtype=storage, device=device)
+    return x0, meas, truth
 
 
 def voxel_config(i: int):

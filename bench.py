@@ -97,6 +97,49 @@ def sirt_seconds_per_iter(iters: int = 20) -> dict:
             "setup_s": t_setup, "iterations": iters, "relative_residual_last": r}
 
 
+def batched_leg(iters: int = 10) -> dict:
+    """64 uneven tilted-detector SIRT problems (32^3-192^3, 60-360 views,
+    64^2-256^2), `iters` iterations each: the plain FP32 GPU form (fp32
+    storage, problems one after another, launched eagerly, default launch
+    shapes) against the batched form (fp16 storage, FP32 arithmetic,
+    autotuned launch shapes, all problems in one HIP graph over 8 streams)."""
+    import torch
+
+    from aprict_amd import phantoms as ph
+    from aprict_amd.autotune import Autotuner, set_variant
+    from aprict_amd.iterators import BatchedSIRT
+
+    c = ph.CONFIGS[5]
+    entries = ph.batched_problems(c["problems"], c["seed"])
+    for op in range(3):
+        set_variant(op, 0)
+    plain = BatchedSIRT([ph.batched_problem_data(e, "float32")[:2] for e in entries], streams=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.run_eager(iters)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    tuner = Autotuner()
+    t0 = time.perf_counter()
+    batched = BatchedSIRT([ph.batched_problem_data(e, "float16")[:2] for e in entries], streams=8, tuner=tuner)
+    batched.capture(iters)
+    torch.cuda.synchronize()
+    t_prep = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batched.run()
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    num = den = 0.0
+    for a, b in zip(batched.volumes, plain.volumes, strict=True):
+        num += float(torch.linalg.vector_norm((a.data.double() - b.data.double())) ** 2)
+        den += float(torch.linalg.vector_norm(b.data.double()) ** 2)
+    voxel_views = sum(e["N"] ** 3 * e["views"] for e in entries)
+    return {"workload": f"{len(entries)} tilted-detector SIRT problems, {iters} iterations each",
+            "s_plain_fp32_eager": t_plain, "s_batched_fp16_graph": t_batched, "speedup": t_plain / t_batched,
+            "gups_batched": 2 * voxel_views * iters / t_batched / 1e9,
+            "prep_s": t_prep, "rel_l2_fp16_vs_fp32": (num / den) ** 0.5}
+
+
 def cpu_baseline(vol_np, pg, n, w, views: int) -> dict:
     """FP32 OpenMP port of the reference semantics (oracle/liboracle32.so):
     the reference's snap DDA forward projector and its exact voxel-driven
@@ -168,6 +211,7 @@ def main() -> int:
     ap.add_argument("--cpu-views", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sirt", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -259,6 +303,9 @@ def main() -> int:
     if not args.no_sirt:
         log("SIRT 256^3 x 360 x 384^2")
         out["sirt"] = sirt_seconds_per_iter()
+    if not args.no_batched:
+        log("batched SIRT: 64 problems, plain eager vs one graph")
+        out["batched"] = batched_leg()
     if not args.no_cpu_baseline:
         log(f"CPU baseline on {args.cpu_views} views")
         vnp = vol.data.float().cpu().numpy()

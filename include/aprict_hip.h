@@ -285,6 +285,33 @@ int32_t ap_sirt_iterate(const ap_config_cb_backproject* bp_cfg,
                         const ap_array3* sino_desc, const void* b, void* r,
                         void* stream);
 
+/* ---- launch-shape tuning -------------------------------------------------
+ * No counterpart in the reference (its launch shapes are fixed).  For the
+ * kernels whose best launch shape depends on the problem size, the library
+ * holds one process-wide variant index per operation; every variant computes
+ * bit-identical results, only the work decomposition differs.  The
+ * project-written autotuner (aprict_amd/autotune.py) times the variants per
+ * problem class and selects one before launching (also while a HIP graph is
+ * being captured: the variant is read when the launch is recorded).  Not
+ * synchronised across host threads: set it from the thread that launches.
+ *   AP_TUNE_PROJECT_RAY     per-ray forward (tilted detectors): pixels per
+ *                           workgroup 64x4, 64x2, 64x1 (gathers per batch
+ *                           4, 4, 8)
+ *   AP_TUNE_PROJECT_COLUMN  column forward (upright detectors): detector rows
+ *                           per workgroup 256, 128, 64
+ *   AP_TUNE_BACKPROJECT     voxel-driven backprojector: voxels per workgroup
+ *                           64x4, 64x2, 64x1                                 */
+#define AP_TUNE_PROJECT_RAY     0
+#define AP_TUNE_PROJECT_COLUMN  1
+#define AP_TUNE_BACKPROJECT     2
+#define AP_TUNE_NUM_OPS         3
+/* number of variants of op (or -1 for an unknown op) */
+int32_t ap_tune_count(int32_t op);
+/* selected variant of op (or -1) */
+int32_t ap_tune_get(int32_t op);
+/* select variant 0 <= variant < ap_tune_count(op) */
+int32_t ap_tune_set(int32_t op, int32_t variant);
+
 #ifdef __cplusplus
 }
 #endif
