@@ -3,27 +3,32 @@
 // (ap_voxelise_forbild) and per-shape bounding boxes (ap_forbild_bboxes).
 //
 // Every primitive is reduced once per block to an affine map M (in FP64)
-// sending it onto a unit reference shape -- the cube [-1,1]^3, the unit ball,
-// or the unit-radius cylinder |z| <= h -- so a ray needs one 3x3 transform and
-// one quadratic or slab solve per shape.  Ray-primitive parameters are FP64
+// sending it onto a reference shape -- the cube [-1,1]^3, the unit ball, the
+// unit-radius cylinder |z| <= h, or the truncated cone |z| <= h,
+// x^2 + y^2 <= (ca + cb z)^2 -- so a ray needs one 3x3 transform and one
+// quadratic or slab solve per shape.  Ray-primitive parameters are FP64
 // (grazing rays cancel catastrophically in FP32); the compositing sweep and
 // accumulation are FP32/FP64 as the output dtype requires.
 //
 // Semantics: cuda/structs/cu/FORBILD_{Box,Sphere,Ellipsoid,Ellipsoid_free,
 // Cylinder,Cylinder_x,Cylinder_y,Cylinder_z,Ellipt_Cyl,Ellipt_Cyl_x,
-// Ellipt_Cyl_y,Ellipt_Cyl_z,ClipAxisSpec}.cu (intervals / membership),
+// Ellipt_Cyl_y,Ellipt_Cyl_z,Cone,Cone_x,Cone_y,Cone_z,ClipAxisSpec}.cu
+// (intervals / membership; a cone's radius runs linearly from r1 at -l/2 to
+// r2 at +l/2 along its axis),
 // FORBILD_Shape.cu:474-540 (last shape wins), CB_project_texture_chunked.cu
 // :421-593 (sub-pixels, multiply_before_adding, overflow counter).
 #include "ap_common.h"
 
 namespace {
 
-enum PrimClass : int { kNone = 0, kCube = 1, kBall = 2, kCyl = 3 };
+enum PrimClass : int { kNone = 0, kCube = 1, kBall = 2, kCyl = 3, kCone = 4 };
 
 struct Prim {
     double M[9];      // rows of the map r -> M (r - c)
     double c[3];
-    double h;         // cylinder half length
+    double h;         // cylinder / cone half length
+    double ca, cb;    // cone radius ca + cb z in the cone frame
+    double cr;        // cone: max(|r1|, |r2|)
     float clip_ax[3][3];
     float clip_d[3];
     int clip_mode[3];
@@ -40,6 +45,7 @@ __device__ void make_prim(const ap_forbild_shape& s, Prim& P) {
     d3 ex = mkd3(1, 0, 0), ey = mkd3(0, 1, 0), ez = mkd3(0, 0, 1);
     double sx = 1, sy = 1, sz = 1;
     P.h = 0;
+    P.ca = P.cb = P.cr = 0;
     P.cls = kNone;
     switch (s.type) {
     case 1: P.cls = kCube; sx = 2.0 / s.dx; sy = 2.0 / s.dy; sz = 2.0 / s.dz; break;
@@ -60,6 +66,20 @@ __device__ void make_prim(const ap_forbild_shape& s, Prim& P) {
         ex = e1; ey = cross(ax, e1); ez = ax;
         sx = sy = 1.0 / s.r; sz = 1.0;
         P.h = 0.5 * (double)s.l;
+        break;
+    }
+    case 2: case 3: case 4: case 5: {
+        P.cls = kCone;
+        d3 ax = s.type == 2 ? unit_of(s.axis)
+              : s.type == 3 ? mkd3(1, 0, 0) : s.type == 4 ? mkd3(0, 1, 0) : mkd3(0, 0, 1);
+        d3 t = fabs(ax.x) < 0.9 ? mkd3(1, 0, 0) : mkd3(0, 1, 0);
+        d3 e1 = cross(ax, t);
+        e1 = e1 * (1.0 / sqrt(dot(e1, e1)));
+        ex = e1; ey = cross(ax, e1); ez = ax;
+        P.h = 0.5 * (double)s.l;
+        P.ca = 0.5 * ((double)s.r1 + (double)s.r2);
+        P.cb = ((double)s.r2 - (double)s.r1) / (double)s.l;
+        P.cr = fmax(fabs((double)s.r1), fabs((double)s.r2));
         break;
     }
     case 12: case 14: case 15: case 16:
@@ -167,6 +187,54 @@ __device__ bool prim_interval(const Prim& P, const d3& S, const d3& d, double& t
         }
         break;
     }
+    case kCone: {
+        // axial slab |z| <= h
+        if (ld[2] == 0.0) {
+            if (fabs(ls[2]) > P.h) return false;
+        } else {
+            const double inv = 1.0 / ld[2];
+            const double x0 = (-P.h - ls[2]) * inv, x1 = (P.h - ls[2]) * inv;
+            ta = fmax(ta, fmin(x0, x1));
+            tb = fmin(tb, fmax(x0, x1));
+        }
+        if (!(tb > ta)) return false;
+        // f(t) = x^2 + y^2 - (ca + cb z)^2 = A t^2 + 2 B t + C.  Inside the slab
+        // only the nappe with ca + cb z >= 0 exists and the truncated cone is
+        // convex, so {f <= 0} meets [ta, tb] in one interval.
+        const double g0 = P.ca + P.cb * ls[2], gd = P.cb * ld[2];
+        const double A = ld[0] * ld[0] + ld[1] * ld[1] - gd * gd;
+        const double B = ls[0] * ld[0] + ls[1] * ld[1] - g0 * gd;
+        const double C = ls[0] * ls[0] + ls[1] * ls[1] - g0 * g0;
+        if (A == 0.0) {
+            if (B == 0.0) {
+                if (C > 0.0) return false;
+            } else if (B > 0.0) {
+                tb = fmin(tb, -C / (2.0 * B));
+            } else {
+                ta = fmax(ta, -C / (2.0 * B));
+            }
+        } else if (A > 0.0) {
+            if (!quad_roots(A, B, C, t0, t1)) return false;
+            ta = fmax(ta, t0);
+            tb = fmin(tb, t1);
+        } else if (quad_roots(A, B, C, t0, t1)) {
+            // f <= 0 outside (t0, t1): keep the piece on the real nappe
+            const double la = ta, lb = fmin(tb, t0), ra = fmax(ta, t1), rb = tb;
+            const bool left = lb > la, right = rb > ra;
+            if (left && right) {
+                const double zl = ls[2] + 0.5 * (la + lb) * ld[2];
+                if (P.ca + P.cb * zl >= 0.0) tb = lb;
+                else ta = ra;
+            } else if (left) {
+                tb = lb;
+            } else if (right) {
+                ta = ra;
+            } else {
+                return false;
+            }
+        }   // A < 0 without real roots: f < 0 on the whole line
+        break;
+    }
     default:
         return false;
     }
@@ -189,6 +257,10 @@ __device__ bool prim_contains(const Prim& P, const d3& p) {
     case kCube: return fabs(a) <= 1.0 && fabs(b) <= 1.0 && fabs(e) <= 1.0;
     case kBall: return a * a + b * b + e * e <= 1.0;
     case kCyl: return fabs(e) <= P.h && a * a + b * b <= 1.0;
+    case kCone: {
+        const double g = P.ca + P.cb * e;
+        return fabs(e) <= P.h && a * a + b * b <= g * g;
+    }
     default: return false;
     }
 }
@@ -403,6 +475,7 @@ __global__ void k_bboxes(const ap_forbild_shape* __restrict__ shapes, uint32_t K
         if (P.cls == kCube) h = fabs(a) + fabs(b) + fabs(e);
         else if (P.cls == kBall) h = sqrt(a * a + b * b + e * e);
         else if (P.cls == kCyl) h = sqrt(a * a + b * b) + P.h * fabs(e);
+        else if (P.cls == kCone) h = P.cr * sqrt(a * a + b * b) + P.h * fabs(e);
         else h = 0.0;
         out[6 * q + w] = (float)(P.c[w] - h);
         out[6 * q + 3 + w] = (float)(P.c[w] + h);

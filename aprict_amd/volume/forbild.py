@@ -4,7 +4,9 @@ Type ids and field meanings follow ``src/aprict/cuda/structs/FORBILD_volumes.py`
 (:502-607): Box ``dx,dy,dz`` are full side lengths; ellipsoid semi-axes
 ``dx,dy,dz`` lie along the normalised ``a_x,a_y,a_z``; elliptic cylinders have
 semi-axes ``dx,dy`` along ``a_x,a_y`` and full length ``l`` along ``axis``;
-clip plane mode 1 keeps ``axis.r <= d``, mode 2 keeps ``axis.r > d``.
+cones have radius ``r1`` at -l/2 and ``r2`` at +l/2 along their axis
+(``cuda/structs/cu/FORBILD_Cone_z.cu:1-13``); clip plane mode 1 keeps
+``axis.r <= d``, mode 2 keeps ``axis.r > d``.
 """
 
 from __future__ import annotations
@@ -18,7 +20,7 @@ TYPE_IDS = {
     "Cylinder_y": 8, "Cylinder_z": 9, "Ellipsoid": 10, "Ellipsoid_free": 11, "Ellipt_Cyl": 12,
     "Sphere": 13, "Ellipt_Cyl_x": 14, "Ellipt_Cyl_y": 15, "Ellipt_Cyl_z": 16,
 }
-SUPPORTED = {1, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16}
+SUPPORTED = set(range(1, 17))
 
 
 def _v(a) -> tuple[float, float, float]:
@@ -84,6 +86,15 @@ def ellipt_cyl(center, dx, dy, l, a_x, a_y, axis, rho, clips=None):  # noqa: E74
 
 def cylinder(center, r, l, axis, rho, clips=None):  # noqa: E741
     return shape("Cylinder", rho=rho, center=center, r=r, l=l, axis=axis, clips=clips)
+
+
+def cone(center, r1, r2, l, axis, rho, clips=None):  # noqa: E741
+    """Truncated cone of length ``l``: radius ``r1`` at -l/2 and ``r2`` at +l/2
+    along ``axis`` ("x", "y", "z" or a vector)."""
+    if isinstance(axis, str):
+        return shape({"x": "Cone_x", "y": "Cone_y", "z": "Cone_z"}[axis], rho=rho, center=center, r1=r1, r2=r2,
+                     l=l, clips=clips)
+    return shape("Cone", rho=rho, center=center, r1=r1, r2=r2, l=l, axis=axis, clips=clips)
 
 
 def stack(shapes) -> np.ndarray:

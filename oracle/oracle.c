@@ -741,6 +741,63 @@ static ival unit_cyl(v3 S, v3 d, real l) {
 
 static v3 unit3(ap_fvec3 a) { v3 r = vf(a); return scl(r, 1 / nrm(r)); }
 
+/* Truncated cone in its frame (axis z): |z| <= l/2 and
+ * x^2 + y^2 <= rad(z)^2, rad(z) = r1 + (z/l + 1/2)(r2 - r1)
+ * (cuda/structs/cu/FORBILD_Cone_z.cu:1-13).  Breakpoints are the slab ends
+ * and the real roots of x^2 + y^2 - rad^2 along the ray; every piece between
+ * consecutive breakpoints is inside or outside as a whole, decided by the
+ * membership test at its midpoint. */
+static int in_cone_frame(v3 p, real l, real r1, real r2) {
+    if (fabs((double)p.z) > (double)l / 2) return 0;
+    real rad = r1 + (p.z / l + (real)0.5) * (r2 - r1);
+    return rad >= 0 && p.x * p.x + p.y * p.y <= rad * rad;
+}
+
+static ival unit_cone(v3 S, v3 d, real l, real r1, real r2) {
+    real br[6];
+    int nb = 0;
+    br[nb++] = 0;
+    br[nb++] = 1;
+    if (d.z != 0) {
+        br[nb++] = (l / 2 - S.z) / d.z;
+        br[nb++] = (-l / 2 - S.z) / d.z;
+    }
+    /* rad(z) = ca + cb z along S + t d */
+    real cb = (r2 - r1) / l, ca = (r1 + r2) / 2;
+    real g0 = ca + cb * S.z, gd = cb * d.z;
+    real A = d.x * d.x + d.y * d.y - gd * gd;
+    real B = 2 * (S.x * d.x + S.y * d.y - g0 * gd);
+    real C = S.x * S.x + S.y * S.y - g0 * g0;
+    if (A != 0) {
+        real disc = B * B - 4 * A * C;
+        if (disc > 0) {
+            real sq = (real)sqrt((double)disc);
+            br[nb++] = (-B - sq) / (2 * A);
+            br[nb++] = (-B + sq) / (2 * A);
+        }
+    } else if (B != 0) {
+        br[nb++] = -C / B;
+    }
+    /* sort the breakpoints inside [0,1] */
+    real pts[6];
+    int np = 0;
+    for (int i = 0; i < nb; i++)
+        if (br[i] >= 0 && br[i] <= 1) pts[np++] = br[i];
+    for (int i = 1; i < np; i++)
+        for (int j = i; j > 0 && pts[j] < pts[j - 1]; j--) { real t = pts[j]; pts[j] = pts[j - 1]; pts[j - 1] = t; }
+    ival r = iv_empty();
+    int found = 0;
+    for (int i = 0; i + 1 < np; i++) {
+        if (!(pts[i + 1] > pts[i])) continue;
+        real mid = (pts[i] + pts[i + 1]) / 2;
+        if (in_cone_frame(add(S, scl(d, mid)), l, r1, r2)) {
+            if (!found) { r.a = pts[i]; found = 1; }
+            r.b = pts[i + 1];
+        }
+    }
+    return r;
+}
+
 static ival clip_plane(const ap_forbild_clip* c, v3 S, v3 d) {
     ival full = {0, 1};
     if (c->mode == 0) return full;
@@ -793,6 +850,17 @@ static ival shape_interval(const ap_forbild_shape* s, v3 S, v3 Q) {
         v3 Sl = mk(dot(Sr, e1) / r, dot(Sr, e2) / r, dot(Sr, ax));
         v3 dl = mk(dot(d, e1) / r, dot(d, e2) / r, dot(d, ax));
         prim = unit_cyl(Sl, dl, s->l);
+        break;
+    }
+    case 2: case 3: case 4: case 5: { /* cone, axis generic/x/y/z */
+        v3 ax = s->type == 2 ? unit3(s->axis)
+              : s->type == 3 ? mk(1, 0, 0) : s->type == 4 ? mk(0, 1, 0) : mk(0, 0, 1);
+        v3 t = fabs((double)ax.x) < 0.9 ? mk(1, 0, 0) : mk(0, 1, 0);
+        v3 e1 = cross(ax, t); e1 = scl(e1, 1 / nrm(e1));
+        v3 e2 = cross(ax, e1);
+        v3 Sl = mk(dot(Sr, e1), dot(Sr, e2), dot(Sr, ax));
+        v3 dl = mk(dot(d, e1), dot(d, e2), dot(d, ax));
+        prim = unit_cone(Sl, dl, s->l, s->r1, s->r2);
         break;
     }
     case 10: { /* axis-aligned ellipsoid, semi-axes dx,dy,dz */
@@ -928,6 +996,13 @@ static int in_shape(const ap_forbild_shape* s, v3 p) {
         if (fabs((double)z) > s->l / 2.0) return 0;
         v3 rr = sub(r, scl(ax, z));
         return dot(rr, rr) <= (real)s->r * s->r;
+    }
+    case 2: case 3: case 4: case 5: {
+        v3 ax = s->type == 2 ? unit3(s->axis)
+              : s->type == 3 ? mk(1, 0, 0) : s->type == 4 ? mk(0, 1, 0) : mk(0, 0, 1);
+        real z = dot(r, ax);
+        v3 rr = sub(r, scl(ax, z));
+        return in_cone_frame(mk(nrm(rr), 0, z), s->l, s->r1, s->r2);
     }
     case 10: {
         real a = r.x / s->dx, b = r.y / s->dy, c = r.z / s->dz;

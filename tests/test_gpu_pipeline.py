@@ -76,6 +76,24 @@ def test_forbild_projector_matches_oracle(device):
     assert err <= 1e-5 and ps.out_of_mem_count == 0
 
 
+def test_forbild_head_phantom_file_matches_oracle(device):
+    # the published head phantom parsed from its .pha file: 479 primitives
+    # (cones and clip planes included), deep overlaps along most rays
+    from aprict_amd.volume import pha
+
+    shapes = pha.convert_ph_file_to_shapes(pathlib.Path(__file__).parent / "golden" / "forbild_head" /
+                                           "HeadPhantom.pha")
+    pg = ph.circular_geometry(16, 64, 64, 0.9, R=500.0, L=1000.0)
+    cfg = cbfp.ConfigProject(ssu=2, ssv=2)
+    ps = cbfp.ProjectionSet(pg)
+    cbfp.project(cfg, FBVolume(shapes), ps, block=True)
+    want = orc.project_forbild(cfg.contents_fb, pg.to_proj_spec_array(), shapes, np.zeros((16, 64, 64)))
+    err = rel_l2(ps.data.cpu().numpy(), want)
+    print(f"FORBILD head phantom ({len(shapes)} primitives) rel L2 {err:.3e}")
+    assert np.abs(want).max() > 10.0
+    assert err <= 1e-5
+
+
 def test_forbild_many_overlaps_use_exact_fallback(device):
     # 60 nested spheres: every central ray meets more shapes than the
     # per-ray interval slots, exercising the recomputing sweep

@@ -93,3 +93,53 @@ def test_voxelised_projection_converges_to_analytic():
     assert np.abs(exact).max() > 1.0
     # first-order convergence in the voxel width
     assert errs[1] < 0.6 * errs[0] and errs[1] < 0.2
+
+
+def test_cone_chords_match_closed_forms():
+    # Cone_z at the origin, radius 1 at z=-2 rising to 3 at z=+2: rad(z) = 2 + z/2
+    cz = forbild.cone((0, 0, 0), 1.0, 3.0, 4.0, "z", 1.0)
+    # perpendicular ray through the axis at height z: chord 2 rad(z)
+    for z in (-1.5, 0.0, 1.0, 1.75):
+        assert integral([cz], (-20, 0, z), (20, 0, z)) == pytest.approx(2 * (2 + z / 2), rel=1e-12)
+    # perpendicular ray at distance 1.25 from the axis at z=1 (rad 2.5)
+    assert integral([cz], (-20, 1.25, 1.0), (20, 1.25, 1.0)) == pytest.approx(2 * np.sqrt(2.5**2 - 1.25**2), rel=1e-12)
+    # ray parallel to the axis at distance 1.5: inside where 2 + z/2 >= 1.5, z in [-1, 2]
+    assert integral([cz], (1.5, 0, -9), (1.5, 0, 9)) == pytest.approx(3.0, rel=1e-12)
+    assert integral([cz], (0.5, 0, 9), (0.5, 0, -9)) == pytest.approx(4.0, rel=1e-12)   # whole length
+    assert integral([cz], (3.5, 0, -9), (3.5, 0, 9)) == 0.0
+    # r1 == r2 is the cylinder
+    cc = forbild.cone((0, 0, 0), 1.5, 1.5, 4.0, "z", 1.0)
+    cy = forbild.cylinder((0, 0, 0), 1.5, 4.0, (0, 0, 1), 1.0)
+    for src, dst in (((-9, 0.4, 0.5), (9, 0.4, 0.5)), ((0.2, 0, -9), (0.2, 0, 9)), ((-5, -3, -4), (6, 2, 5))):
+        assert integral([cc], src, dst) == pytest.approx(integral([cy], src, dst), rel=1e-12)
+
+
+def test_cone_axes_agree_and_oblique_rays_match_sampling():
+    rng = np.random.default_rng(8)
+    base = dict(r1=0.6, r2=2.2, l=3.0, rho=1.0)
+    c = np.array([0.3, -0.2, 0.4])
+    # world coordinates of a cone-frame point (axis = local z) for each axis type
+    to_world = {"x": [2, 0, 1], "y": [0, 2, 1], "z": [0, 1, 2]}
+    for _ in range(4):
+        p0 = rng.uniform(-4, 4, 3)
+        p1 = -p0 + rng.uniform(-1, 1, 3)
+        ref0 = integral([forbild.cone((0, 0, 0), base["r1"], base["r2"], base["l"], "z", 1.0)],
+                        tuple(p0), tuple(p1))
+        for ax, idx in to_world.items():
+            got = integral([forbild.cone((0, 0, 0), base["r1"], base["r2"], base["l"], ax, 1.0)],
+                           tuple(p0[idx]), tuple(p1[idx]))
+            assert got == pytest.approx(ref0, rel=1e-12, abs=1e-12), ax
+        ref = integral([forbild.cone(tuple(c), base["r1"], base["r2"], base["l"], "z", 1.0)],
+                       tuple(c + p0), tuple(c + p1))
+        # a generic axis: the z-frame ray rotated with the cone
+        rot = ph.haar_rotation(rng)
+        got = integral([forbild.cone(tuple(c), base["r1"], base["r2"], base["l"], tuple(rot[:, 2]), 1.0)],
+                       tuple(c + rot @ p0), tuple(c + rot @ p1))
+        assert got == pytest.approx(ref, rel=1e-6, abs=1e-9)
+        # dense membership sampling along the ray (independent of the interval solver)
+        n = 400000
+        t = (np.arange(n) + 0.5) / n
+        pts = p0[None, :] + t[:, None] * (p1 - p0)[None, :]
+        rad = base["r1"] + (pts[:, 2] / base["l"] + 0.5) * (base["r2"] - base["r1"])
+        inside = (np.abs(pts[:, 2]) <= base["l"] / 2) & (pts[:, 0] ** 2 + pts[:, 1] ** 2 <= rad**2)
+        assert ref == pytest.approx(inside.mean() * np.linalg.norm(p1 - p0), abs=2e-4)
