@@ -48,6 +48,30 @@ def test_matched_adjoint_matches_oracle(device):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("pitch, ss", [(0.9, 1), (0.1, 2)], ids=["coarse_32bit", "fine_64bit"])
+def test_column_adjoint_accumulation_modes(device, pitch, ss):
+    # upright views (the column kernel).  A coarse detector puts a few rays
+    # through each voxel per view and the bricks accumulate in 32-bit fixed
+    # point with periodic flushes; a detector ten times finer than the
+    # voxels' shadow, sub-sampled 2x2, puts hundreds through each voxel, and
+    # the bricks fall back to 64-bit accumulation.
+    rng = np.random.default_rng(17)
+    n, w = 24, 0.5
+    nd = 16 if pitch > 0.5 else 128
+    pg = ph.circular_geometry(10, nd, nd, pitch, R=20.0, L=40.0)
+    y = rng.uniform(-1, 2, (10, nd, nd)).astype(np.float32)
+    cfg = cbfp.ConfigProject(ssu=ss, ssv=ss)
+    v = Volume(np.zeros((n, n, n), np.float32), voxwid=w)
+    ps = cbfp.ProjectionSet(pg, data=to_gpu(y))
+    cbfp.backproject_matched(cfg, v, ps, block=True)
+    want = orc.backproject_matched(cfg.contents, pg.to_proj_spec_array(), chunk_for((n, n, n), w),
+                                   np.zeros((n, n, n)), y)
+    err = rel_l2(v.as_numpy(), want)
+    print(f"column adjoint pitch {pitch} ss {ss}: rel L2 {err:.3e}")
+    assert np.abs(want).max() > 0
+    assert err <= 1e-5
+
+
 def _dot_test(pg, n, w, seed):
     import torch
 
