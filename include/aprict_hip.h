@@ -66,7 +66,14 @@ typedef struct {
 
 /* vol_tex_chunk_spec: cuda/structs/geometry.py:175-186 (40 bytes).
  * texpos = corner of voxel (0,0,0) of the whole volume; offsets/shape
- * select the chunk (in voxels) that a call reads or writes. */
+ * select the chunk (in voxels) that a call reads or writes.  Every entry
+ * point addresses a volume (and a multiplier) as
+ *   vol + (i*strides[0] + j*strides[1] + k*strides[2]) * itemsize
+ * with whole-volume indices (i,j,k) and touches no element outside the
+ * chunk, so only the chunk needs to be backed by memory: a caller streaming
+ * a volume larger than device memory passes, for a brick buffer holding the
+ * chunk, vol = brick - (offsets . strides) * itemsize and the brick's own
+ * strides. */
 typedef struct {
     float     voxwid;
     ap_uivec3 offsets;
@@ -122,9 +129,9 @@ typedef struct {
 } ap_forbild_clip;
 
 /* FORBILD_Shape: cuda/structs/FORBILD_volumes.py:537-607 (156 bytes).
- * type ids: FORBILD_volumes.py:517-534 (1 Box, 6-9 Cylinder[_x/_y/_z],
- * 10 Ellipsoid, 11 Ellipsoid_free, 12 Ellipt_Cyl, 13 Sphere,
- * 14-16 Ellipt_Cyl_x/_y/_z; 2-5 cones are not supported). */
+ * type ids: FORBILD_volumes.py:517-534 (1 Box, 2-5 Cone[_x/_y/_z],
+ * 6-9 Cylinder[_x/_y/_z], 10 Ellipsoid, 11 Ellipsoid_free, 12 Ellipt_Cyl,
+ * 13 Sphere, 14-16 Ellipt_Cyl_x/_y/_z). */
 typedef struct {
     uint8_t         type;
     float           rho;
