@@ -27,6 +27,7 @@ constexpr int kColRows[3] = {256, 128, 64};
 struct SinoCfgC {
     uint32_t V, nu, nv;
     int64_t s0, s1, s2;
+    uint32_t vec4;     // f32, v contiguous, rows 16-byte aligned, nv % 4 == 0
 };
 
 struct ColMap {
@@ -468,6 +469,24 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
         int j0, j1, k0, k1;
         if (!brick_footprint(make_view_rays(rc, specs, p), blo, bhi, cen, sc.nu, sc.nv, lane, j0, j1, k0, k1))
             continue;
+        if constexpr (std::is_same<Tp, float>::value) {
+            if (sc.vec4) {
+                // rows [k0 & ~3, k1 | 3] as aligned float4 groups (nv % 4 == 0
+                // keeps them inside the detector column); the extra rows only
+                // loosen the bound
+                const int ka = k0 & ~3, ng = ((k1 | 3) - ka + 1) >> 2, ngrp = (j1 - j0 + 1) * ng;
+                const UDiv dg = make_udiv(ng);
+                for (int e = lane; e < ngrp; e += 64) {
+                    const int jj = udiv(e, dg);
+                    const float4 y4 = *reinterpret_cast<const float4*>(
+                        sino + (int64_t)p * sc.s0 + (int64_t)(j0 + jj) * sc.s1 + ka + 4 * (e - jj * ng));
+                    ymax = fmaxf(ymax, fmaxf(fmaxf(fabsf(y4.x), fabsf(y4.y)), fmaxf(fabsf(y4.z), fabsf(y4.w))));
+                    const float sum = (y4.x + y4.y) + (y4.z + y4.w);
+                    if (sum != sum) ymax = __builtin_inff();   // NaN (or opposite infinities)
+                }
+                continue;
+            }
+        }
         const int nk = k1 - k0 + 1, npix = (j1 - j0 + 1) * nk;
         const UDiv dk = make_udiv(nk);
         for (int e = lane; e < npix; e += 64) {
@@ -573,28 +592,25 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
             // the sinogram value of the lane's next pair is fetched one pair ahead
             const int npair = nc * nrow;
             const UDiv drow = make_udiv(nrow), dsv = make_udiv((int)ssv);
-            auto yval = [&](int e) -> float {
-                if (e >= npair) return 0.f;
+            auto yval = [&](int e0) -> float {
+                const int e = min(e0, npair - 1);
                 const int g = udiv(e, drow);
                 const int k = k0 + udiv(e - g * nrow, dsv);
                 return to_f32(sino[(int64_t)p * sc.s0 + (int64_t)W.ce[g].j * sc.s1 + (int64_t)k * sc.s2]);
             };
-            float y_next = yval(lane);
-            for (int e = lane; e < npair; e += 64) {
-                const float y = y_next;
-                y_next = yval(e + 64);
+            auto pair = [&](int e, float y) {
                 const int g = udiv(e, drow), kk = e - g * nrow;
                 const ColEntry& E = W.ce[g];
-                if (E.nseg == 0 || y == 0.f) continue;
+                if (E.nseg == 0 || y == 0.f) return;
                 const int kq = udiv(kk, dsv);
                 const int k = k0 + kq;
                 LaneZ Z;
                 if (!lane_setup(E.z, (float)k + sub_off((uint32_t)(kk - kq * (int)ssv), ssv), rc.lo[2], rc.hi[2], Z))
-                    continue;
+                    return;
                 float zin, zout;
                 lane_box(Z, blo[2], bhi[2], zin, zout);
                 const float t0 = fmaxf(E.h0, zin), t1 = fminf(E.h1, zout);
-                if (!(t0 < t1)) continue;
+                if (!(t0 < t1)) return;
                 const int iz = axis_index_at(Z.r, Z.c, Z.P0, Z.u, Z.sg, Z.ifix, t0, blo[2], bhi[2]);
                 float pz = Z.sg > 0 ? (float)(iz + 1) - Z.P0 : (float)iz - Z.P0;
                 float hz = Z.sg == 0 ? AP_BIG : tcross(pz, Z.r, Z.c);
@@ -603,8 +619,9 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                 const float wys = y * scaleS * Z.sec;
                 const float* he = W.he[g];
                 const int* ob = W.ob[g];
-                int s0 = 0;
-                while (he[s0] <= t0) s0++;   // he[nseg-1] = h1 > t0
+                // first segment ending after t0 (he[nseg-1] = h1 > t0): a binary
+                // search, a few dependent LDS reads instead of one per segment
+                const int s0 = first_above(he, E.nseg, t0);
                 float hp = t0;
                 if (Z.u > -kSingleSlope && Z.u < kSingleSlope) {
                     // at most one z crossing per segment: 4 segments per step
@@ -687,6 +704,17 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                         if (cend >= t1) break;
                     }
                 }
+            };
+            // two pairs per turn with alternating registers for the value
+            // fetched ahead (the load is unconditional, from a clamped index),
+            // so no register copy or branch makes a use wait for the newer load
+            float ya = yval(lane);
+            for (int e = lane; e < npair; e += 128) {
+                const float yb = yval(e + 64);
+                pair(e, ya);
+                if (e + 64 >= npair) break;
+                ya = yval(e + 128);
+                pair(e + 64, yb);
             }
         }
     }
@@ -720,7 +748,7 @@ int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32
     const bool bytes32 = !idx64 && (uint64_t)(array3_max_offset(vol_desc) + 1) * esz <= 0xFFFFFFFFull;
     const size_t shm = ((cap * sizeof(float) + 15) & ~size_t(15)) + cap * (bytes32 ? 4 : 8);
     if (shm > 60 * 1024) return AP_ERR_UNSUPPORTED;
-    SinoCfgC sc{V, nu, nv, s0, s1, s2};
+    SinoCfgC sc{V, nu, nv, s0, s1, s2, 0u};
     const int var = ap_tune_variant(AP_TUNE_PROJECT_COLUMN);
     const uint32_t rows = (uint32_t)kColRows[var];
     ColMap cm;
@@ -771,7 +799,9 @@ int32_t launch_backproject_columns(const RayCfg& rc, uint32_t V, uint32_t nu, ui
                                    const ap_proj_spec* specs, const ap_array3* vol_desc, void* vol,
                                    int32_t mdt, const int64_t* ms, const void* mult, int32_t sino_dtype,
                                    const void* sino, bool idx64, hipStream_t s) {
-    SinoCfgC sc{V, nu, nv, s0, s1, s2};
+    SinoCfgC sc{V, nu, nv, s0, s1, s2, 0u};
+    sc.vec4 = sino_dtype == AP_F32 && s2 == 1 && nv % 4 == 0 && s1 % 4 == 0 && s0 % 4 == 0 &&
+              ((uintptr_t)sino & 15) == 0;
     // the 64-bit fixed-point sums stay exact while views x rays per view < 2^33
     if ((double)V * nu * nv * ssu * ssv >= 8589934592.0) return AP_ERR_UNSUPPORTED;
     const uint32_t nbx = (uint32_t)(rc.hi[0] - rc.lo[0] + kTX - 1) / kTX;

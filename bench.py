@@ -97,6 +97,38 @@ def sirt_seconds_per_iter(iters: int = 20) -> dict:
             "setup_s": t_setup, "iterations": iters, "relative_residual_last": r}
 
 
+def storage_leg(P, x32, y32, g32, t_fp32: float, steps: int = 3) -> dict:
+    """The same forward + adjoint step with 16-bit storage: bf16 volume, fp16
+    sinogram, bf16 adjoint output (FP32 arithmetic in the kernels), timed like
+    the fp32 step, with its error against the fp32 results of the same input."""
+    import torch
+
+    xb = x32.to(torch.bfloat16)
+    yh = torch.zeros(P.sino_shape, dtype=torch.float16, device="cuda")
+    gb = torch.zeros(P.vol_shape, dtype=torch.bfloat16, device="cuda")
+    stream = torch.cuda.current_stream()
+    rec = []
+    for it in range(steps + 1):
+        yh.zero_()
+        gb.zero_()
+        ea = time_events(lambda: P.A(xb, out=yh), stream)
+        eb = time_events(lambda: P.AT(yh, out=gb), stream)
+        if it > 0:
+            rec.append((ea, eb))
+    torch.cuda.synchronize()
+    t_a = float(np.mean([ea[0].elapsed_time(ea[1]) for ea, _ in rec])) / 1e3
+    t_at = float(np.mean([eb[0].elapsed_time(eb[1]) for _, eb in rec])) / 1e3
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm((a.double() - b.double()).flatten())
+                     / torch.linalg.vector_norm(b.double().flatten()))
+
+    return {"workload": "same step, bf16 volume / fp16 sinogram / bf16 adjoint output, fp32 arithmetic",
+            "ms_forward": t_a * 1e3, "ms_adjoint": t_at * 1e3,
+            "speedup_vs_fp32": t_fp32 / (t_a + t_at),
+            "rel_l2_forward_vs_fp32": rel(yh, y32), "rel_l2_adjoint_vs_fp32": rel(gb, g32)}
+
+
 def batched_leg(iters: int = 10) -> dict:
     """64 uneven tilted-detector SIRT problems (32^3-192^3, 60-360 views,
     64^2-256^2), `iters` iterations each: the plain FP32 GPU form (fp32
@@ -212,6 +244,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sirt", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-storage", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -300,6 +333,9 @@ def main() -> int:
                      "model": "B_op = V (N^3 s_v + nu nv s_p) bytes per operator application; time = the "
                               "operator call on its stream (HIP events)"},
     }
+    if args.volume_dtype == "f32" and not args.no_storage:
+        log("16-bit storage leg")
+        out["bf16_storage"] = storage_leg(P, x, y, g, t_a + t_at)
     if not args.no_sirt:
         log("SIRT 256^3 x 360 x 384^2")
         out["sirt"] = sirt_seconds_per_iter()

@@ -48,13 +48,12 @@ def test_matched_adjoint_matches_oracle(device):
     assert err <= 1e-5
 
 
-@pytest.mark.parametrize("pitch, ss", [(0.9, 1), (0.1, 2)], ids=["coarse_32bit", "fine_64bit"])
-def test_column_adjoint_accumulation_modes(device, pitch, ss):
+@pytest.mark.parametrize("pitch, ss", [(0.9, 1), (0.1, 2)], ids=["coarse", "fine"])
+def test_column_adjoint_coarse_and_fine_detectors(device, pitch, ss):
     # upright views (the column kernel).  A coarse detector puts a few rays
-    # through each voxel per view and the bricks accumulate in 32-bit fixed
-    # point with periodic flushes; a detector ten times finer than the
-    # voxels' shadow, sub-sampled 2x2, puts hundreds through each voxel, and
-    # the bricks fall back to 64-bit accumulation.
+    # through each voxel per view; a detector ten times finer than the
+    # voxels' shadow, sub-sampled 2x2, puts hundreds through each voxel (the
+    # fixed-point scale must hold many terms per voxel and view).
     rng = np.random.default_rng(17)
     n, w = 24, 0.5
     nd = 16 if pitch > 0.5 else 128
