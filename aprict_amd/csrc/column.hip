@@ -27,7 +27,7 @@ constexpr int kColRows[3] = {256, 128, 64};
 struct SinoCfgC {
     uint32_t V, nu, nv;
     int64_t s0, s1, s2;
-    uint32_t vec4;     // f32, v contiguous, rows 16-byte aligned, nv % 4 == 0
+    uint32_t vec16;    // f32 or f16, v contiguous, rows 16-byte aligned, nv a multiple of 16 bytes
 };
 
 struct ColMap {
@@ -469,23 +469,30 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
         int j0, j1, k0, k1;
         if (!brick_footprint(make_view_rays(rc, specs, p), blo, bhi, cen, sc.nu, sc.nv, lane, j0, j1, k0, k1))
             continue;
-        if constexpr (std::is_same<Tp, float>::value) {
-            if (sc.vec4) {
-                // rows [k0 & ~3, k1 | 3] as aligned float4 groups (nv % 4 == 0
-                // keeps them inside the detector column); the extra rows only
-                // loosen the bound
-                const int ka = k0 & ~3, ng = ((k1 | 3) - ka + 1) >> 2, ngrp = (j1 - j0 + 1) * ng;
-                const UDiv dg = make_udiv(ng);
-                for (int e = lane; e < ngrp; e += 64) {
-                    const int jj = udiv(e, dg);
-                    const float4 y4 = *reinterpret_cast<const float4*>(
-                        sino + (int64_t)p * sc.s0 + (int64_t)(j0 + jj) * sc.s1 + ka + 4 * (e - jj * ng));
-                    ymax = fmaxf(ymax, fmaxf(fmaxf(fabsf(y4.x), fabsf(y4.y)), fmaxf(fabsf(y4.z), fabsf(y4.w))));
-                    const float sum = (y4.x + y4.y) + (y4.z + y4.w);
-                    if (sum != sum) ymax = __builtin_inff();   // NaN (or opposite infinities)
+        if (sc.vec16) {
+            // rows [k0, k1] widened to aligned 16-byte groups of kVec values
+            // (nv % kVec == 0 keeps them inside the detector column); the
+            // extra rows only loosen the bound
+            constexpr int kVec = 16 / (int)sizeof(Tp);
+            const int ka = k0 & ~(kVec - 1), ng = ((k1 | (kVec - 1)) - ka + 1) / kVec;
+            const int ngrp = (j1 - j0 + 1) * ng;
+            const UDiv dg = make_udiv(ng);
+            for (int e = lane; e < ngrp; e += 64) {
+                const int jj = udiv(e, dg);
+                const uint4 raw = *reinterpret_cast<const uint4*>(
+                    sino + (int64_t)p * sc.s0 + (int64_t)(j0 + jj) * sc.s1 + ka + kVec * (e - jj * ng));
+                const Tp* v = reinterpret_cast<const Tp*>(&raw);
+                float m = 0.f, sum = 0.f;
+#pragma unroll
+                for (int q = 0; q < kVec; q++) {
+                    const float y = to_f32(v[q]);
+                    m = fmaxf(m, fabsf(y));
+                    sum += y;
                 }
-                continue;
+                ymax = fmaxf(ymax, m);
+                if (sum != sum) ymax = __builtin_inff();   // NaN (or opposite infinities)
             }
+            continue;
         }
         const int nk = k1 - k0 + 1, npix = (j1 - j0 + 1) * nk;
         const UDiv dk = make_udiv(nk);
@@ -800,8 +807,11 @@ int32_t launch_backproject_columns(const RayCfg& rc, uint32_t V, uint32_t nu, ui
                                    int32_t mdt, const int64_t* ms, const void* mult, int32_t sino_dtype,
                                    const void* sino, bool idx64, hipStream_t s) {
     SinoCfgC sc{V, nu, nv, s0, s1, s2, 0u};
-    sc.vec4 = sino_dtype == AP_F32 && s2 == 1 && nv % 4 == 0 && s1 % 4 == 0 && s0 % 4 == 0 &&
-              ((uintptr_t)sino & 15) == 0;
+    {
+        const int64_t kv = sino_dtype == AP_F32 ? 4 : sino_dtype == AP_F16 ? 8 : 0;
+        sc.vec16 = kv > 0 && s2 == 1 && nv % kv == 0 && s1 % kv == 0 && s0 % kv == 0 &&
+                   ((uintptr_t)sino & 15) == 0;
+    }
     // the 64-bit fixed-point sums stay exact while views x rays per view < 2^33
     if ((double)V * nu * nv * ssu * ssv >= 8589934592.0) return AP_ERR_UNSUPPORTED;
     const uint32_t nbx = (uint32_t)(rc.hi[0] - rc.lo[0] + kTX - 1) / kTX;
