@@ -599,21 +599,24 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
             // the sinogram value of the lane's next pair is fetched one pair ahead
             const int npair = nc * nrow;
             const UDiv drow = make_udiv(nrow), dsv = make_udiv((int)ssv);
+            // one sub-row per pixel (ssv == 1, wave-uniform): the sub-row is the
+            // row and its offset is 0 (what sub_off(0, 1) gives), no divisions
+            const bool ss1 = ssv == 1;
             auto yval = [&](int e0) -> float {
                 const int e = min(e0, npair - 1);
                 const int g = udiv(e, drow);
-                const int k = k0 + udiv(e - g * nrow, dsv);
+                const int k = k0 + (ss1 ? e - g * nrow : udiv(e - g * nrow, dsv));
                 return to_f32(sino[(int64_t)p * sc.s0 + (int64_t)W.ce[g].j * sc.s1 + (int64_t)k * sc.s2]);
             };
             auto pair = [&](int e, float y) {
                 const int g = udiv(e, drow), kk = e - g * nrow;
                 const ColEntry& E = W.ce[g];
                 if (E.nseg == 0 || y == 0.f) return;
-                const int kq = udiv(kk, dsv);
+                const int kq = ss1 ? kk : udiv(kk, dsv);
                 const int k = k0 + kq;
+                const float so = ss1 ? 0.f : sub_off((uint32_t)(kk - kq * (int)ssv), ssv);
                 LaneZ Z;
-                if (!lane_setup(E.z, (float)k + sub_off((uint32_t)(kk - kq * (int)ssv), ssv), rc.lo[2], rc.hi[2], Z))
-                    return;
+                if (!lane_setup(E.z, (float)k + so, rc.lo[2], rc.hi[2], Z)) return;
                 float zin, zout;
                 lane_box(Z, blo[2], bhi[2], zin, zout);
                 const float t0 = fmaxf(E.h0, zin), t1 = fminf(E.h1, zout);
