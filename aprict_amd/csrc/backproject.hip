@@ -28,6 +28,9 @@ struct ViewMap {
     float Zc[3], z0, Lnorm;
     float S[3], D[3], U[3], Vv[3];
     float nu, nv, uvnorm;
+    float kq;              // K^2 / (|u| |v|): the FDK weight is kq w^3 / (W.x + w0)^2
+    float hu, hv;          // (nu - 1) / 2, (nv - 1) / 2: pixel index of the detector centre
+    float inv_hnu, inv_hnv;   // 2 / nu, 2 / nv
 };
 
 __device__ void make_view_map(const ap_proj_spec* specs, uint32_t p, ViewMap& m) {
@@ -58,11 +61,19 @@ __device__ void make_view_map(const ap_proj_spec* specs, uint32_t p, ViewMap& m)
     m.nu = (float)g.nu;
     m.nv = (float)g.nv;
     m.uvnorm = (float)(sqrt(uu) * sqrt(vv));
+    m.kq = (float)(K * K / (sqrt(uu) * sqrt(vv)));
+    m.hu = (float)(0.5 * ((double)g.nu - 1.0));
+    m.hv = (float)(0.5 * ((double)g.nv - 1.0));
+    m.inv_hnu = (float)(2.0 / (double)g.nu);
+    m.inv_hnv = (float)(2.0 / (double)g.nv);
 }
 
 struct BpCfg {
     bool exact, all1s, mask, direct, bilinear, use_mult;
     float mask_r2, sep, zval;
+    float inv_sep;        // 1 / sep (0 when sep == 0)
+    float w3;             // voxwid^3
+    bool off32;           // detector offsets of one view fit 32 bits
     int zl, zv;
     uint32_t ssx, ssy, ssz;
     float w;              // voxwid
@@ -106,9 +117,77 @@ __device__ __forceinline__ float ray_cube_len(const float S[3], const float P[3]
     return t1 > t0 ? (t1 - t0) * sqrtf(d2) : 0.f;
 }
 
+// Detector-edge softening (CB_backproject.cu:694-713): a cosine ramp over the
+// outer `sep` fraction of each half of the detector.  Outside that band both
+// factors are exactly 1 (cos 0 = 1), so the cosines are evaluated only for
+// voxels that project into it.
+__device__ __forceinline__ float soften(const BpCfg& c, const ViewMap& m, float cu, float cv, float val) {
+    if (!(c.sep > 0.f) || val == 0.f) return val;
+    float ud = fabsf(cu) * m.inv_hnu, vd = fabsf(cv) * m.inv_hnv;
+    ud = fmaxf(0.f, fminf(1.f, (1.f - ud) * c.inv_sep));
+    vd = fmaxf(0.f, fminf(1.f, (1.f - vd) * c.inv_sep));
+    if (ud < 1.f || vd < 1.f) {
+        const float pi = 3.14159265358979f;
+        ud = fmaxf(0.f, fminf(1.f, 0.5f * (1.f + cosf(pi * (1.f - ud)))));
+        vd = fmaxf(0.f, fminf(1.f, 0.5f * (1.f + cosf(pi * (1.f - vd)))));
+        val *= ud * vd;
+    }
+    return val;
+}
+
+// The interpolating branch (CB_backproject.cu:478-691): the voxel centre's
+// projection, weighted by the FDK magnification factor (or 1 with direct
+// ray-casting) and Z powers, read bilinearly or at the nearest pixel.  The
+// reciprocal of the projective denominator is the hardware one (within an
+// ulp of a division).  Taps outside the detector read 0.
+template <typename Tp>
+__device__ __forceinline__ float bp_fast_view(const BpCfg& c, const ViewMap& m, const Tp* __restrict__ img,
+                                              const float x[3]) {
+    const float wx = m.W[0] * x[0] + m.W[1] * x[1] + m.W[2] * x[2] + m.w0;
+    const float iw = __builtin_amdgcn_rcpf(wx);
+    const float cu = (m.Au[0] * x[0] + m.Au[1] * x[1] + m.Au[2] * x[2] + m.au) * iw;
+    const float cv = (m.Av[0] * x[0] + m.Av[1] * x[1] + m.Av[2] * x[2] + m.av) * iw;
+    float mult = c.direct ? 1.f : m.kq * c.w3 * (iw * iw);
+    if (c.zl != 0 || c.zv != 0) {
+        const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
+        if (c.zl != 0) mult *= powi(zz / m.Lnorm, c.zl);
+        if (c.zv != 0) mult *= powi(zz / c.zval, c.zv);
+    }
+    if (mult == 0.f) return 0.f;
+    auto rd = [&](int j, int k) -> float {
+        if (c.off32) return to_f32(img[j * (int32_t)c.s1 + k * (int32_t)c.s2]);
+        return to_f32(img[(int64_t)j * c.s1 + (int64_t)k * c.s2]);
+    };
+    const float fj = cu + m.hu, fk = cv + m.hv;
+    float y = 0.f;
+    if (c.bilinear) {
+        if (fj > -1.f && fk > -1.f && fj < (float)c.nu && fk < (float)c.nv) {
+            const float fl_j = floorf(fj), fl_k = floorf(fk);
+            const int j0 = (int)fl_j, k0 = (int)fl_k;
+            const float a = fj - fl_j, b = fk - fl_k;
+            // the four taps at clamped (in-range) indices, zeroed off the detector
+            const int ja = max(j0, 0), jb = min(j0 + 1, (int)c.nu - 1);
+            const int ka = max(k0, 0), kb = min(k0 + 1, (int)c.nv - 1);
+            float y00 = rd(ja, ka), y10 = rd(jb, ka), y01 = rd(ja, kb), y11 = rd(jb, kb);
+            if (j0 < 0) { y00 = 0.f; y01 = 0.f; }
+            if (j0 + 1 >= (int)c.nu) { y10 = 0.f; y11 = 0.f; }
+            if (k0 < 0) { y00 = 0.f; y10 = 0.f; }
+            if (k0 + 1 >= (int)c.nv) { y01 = 0.f; y11 = 0.f; }
+            const float t1 = y00 * (1.f - a) + y10 * a;
+            const float t2 = y01 * (1.f - a) + y11 * a;
+            y = t1 * (1.f - b) + t2 * b;
+        }
+    } else {
+        const int j = (int)floorf(fj + 0.5f), k = (int)floorf(fk + 0.5f);
+        if (j >= 0 && k >= 0 && j < (int)c.nu && k < (int)c.nv) y = rd(j, k);
+    }
+    return soften(c, m, cu, cv, y * mult);
+}
+
 template <typename Tp>
 __device__ float bp_one_view(const BpCfg& c, const ViewMap& m, const Tp* __restrict__ img,
                              const float x[3]) {
+    if (!c.exact) return bp_fast_view(c, m, img, x);
     const float wx = m.W[0] * x[0] + m.W[1] * x[1] + m.W[2] * x[2] + m.w0;
     const float iw = 1.f / wx;
     const float cu = (m.Au[0] * x[0] + m.Au[1] * x[1] + m.Au[2] * x[2] + m.au) * iw;
@@ -152,56 +231,8 @@ __device__ float bp_one_view(const BpCfg& c, const ViewMap& m, const Tp* __restr
             const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
             val *= powi(zz / c.zval, c.zv);
         }
-    } else {
-        float mult;
-        if (c.direct) {
-            mult = 1.f;
-            if (c.zl != 0 || c.zv != 0) {
-                const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
-                if (c.zl != 0) mult *= powi(zz / m.Lnorm, c.zl);
-                if (c.zv != 0) mult *= powi(zz / c.zval, c.zv);
-            }
-        } else {
-            // (|det-src| / |vox-src| * voxwid)^2 / (|u||v|) * voxwid
-            const float lam = m.K * iw;
-            mult = lam * lam * c.w * c.w / m.uvnorm;
-            if (c.zl != 0 || c.zv != 0) {
-                const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
-                if (c.zl != 0) mult *= powi(zz / m.Lnorm, c.zl);
-                if (c.zv != 0) mult *= powi(zz / c.zval, c.zv);
-            }
-            mult *= c.w;
-        }
-        if (mult != 0.f) {
-            const float fj = cu + 0.5f * (m.nu - 1.f), fk = cv + 0.5f * (m.nv - 1.f);
-            float y;
-            if (c.bilinear) {
-                if (fj <= -1.f || fk <= -1.f || fj >= (float)c.nu || fk >= (float)c.nv) {
-                    y = 0.f;
-                } else {
-                    const float fl_j = floorf(fj), fl_k = floorf(fk);
-                    const int j0 = (int)fl_j, k0 = (int)fl_k;
-                    const float a = fj - fl_j, b = fk - fl_k;
-                    const float t1 = det_read(img, c, j0, k0) * (1.f - a) + det_read(img, c, j0 + 1, k0) * a;
-                    const float t2 = det_read(img, c, j0, k0 + 1) * (1.f - a) + det_read(img, c, j0 + 1, k0 + 1) * a;
-                    y = t1 * (1.f - b) + t2 * b;
-                }
-            } else {
-                y = det_read(img, c, (int)floorf(fj + 0.5f), (int)floorf(fk + 0.5f));
-            }
-            val = y * mult;
-        }
     }
-    if (c.sep > 0.f && val != 0.f) {
-        float ud = fabsf(cu) / (0.5f * m.nu), vd = fabsf(cv) / (0.5f * m.nv);
-        ud = fmaxf(0.f, fminf(1.f, (1.f - ud) / c.sep));
-        vd = fmaxf(0.f, fminf(1.f, (1.f - vd) / c.sep));
-        const float pi = 3.14159265358979f;
-        ud = fmaxf(0.f, fminf(1.f, 0.5f * (1.f + cosf(pi * (1.f - ud)))));
-        vd = fmaxf(0.f, fminf(1.f, 0.5f * (1.f + cosf(pi * (1.f - vd)))));
-        val *= ud * vd;
-    }
-    return val;
+    return soften(c, m, cu, cv, val);
 }
 
 template <int kBy, typename Tv, typename Tp, typename Tm>
@@ -232,8 +263,13 @@ __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
         __syncthreads();
         if (!active) continue;
         if (nss == 1) {
-            for (uint32_t q = 0; q < nb; q++)
-                acc1 += bp_one_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
+            if (c.exact) {
+                for (uint32_t q = 0; q < nb; q++)
+                    acc1 += bp_one_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
+            } else {
+                for (uint32_t q = 0; q < nb; q++)
+                    acc1 += bp_fast_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
+            }
         }
     }
     if (nss != 1 && active) {
@@ -298,6 +334,10 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
     c.nu = sino_desc->shape[1];
     c.nv = sino_desc->shape[2];
     c.s0 = sino_desc->strides[0]; c.s1 = sino_desc->strides[1]; c.s2 = sino_desc->strides[2];
+    c.inv_sep = c.sep > 0.f ? 1.f / c.sep : 0.f;
+    c.w3 = c.w * c.w * c.w;
+    c.off32 = (double)(c.nu ? c.nu - 1 : 0) * (double)c.s1 + (double)(c.nv ? c.nv - 1 : 0) * (double)c.s2 <
+              2147483647.0;
     const int var = ap_tune_variant(AP_TUNE_BACKPROJECT);
     const uint32_t rows_y = (uint32_t)kBpRowsY[var];
     dim3 block(kBz, rows_y, 1);
