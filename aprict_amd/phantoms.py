@@ -12,6 +12,7 @@ from __future__ import annotations
 import numpy as np
 
 from .cb import flatpanel as cbfp
+from .forbild_presets import PhantomContainer as FORBILD  # noqa: F401  (aprict.phantoms.FORBILD)
 from .volume import forbild
 
 

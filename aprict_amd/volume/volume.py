@@ -355,6 +355,59 @@ class FBVolume:
             return self
         return FBVolume(self._shapes[keep])
 
+    # ---- phantom editing (volume/volume.py:509-624) --------------------------
+    def _replace(self, shapes: np.ndarray) -> None:
+        self._shapes = np.ascontiguousarray(shapes)
+        self._dev = {}
+        self._bboxes = None
+
+    def scale_len(self, factor: float) -> None:
+        """Scale every length of the phantom (centres, sizes, clip-plane offsets) by ``factor``."""
+        s = self._shapes.copy()
+        for key in ("x", "y", "z", "r", "l", "dx", "dy", "dz", "r1", "r2"):
+            s[key] *= factor
+        for cas in ("cas1", "cas2", "cas3"):
+            s[cas]["dotprod"] *= factor
+        self._replace(s)
+
+    @staticmethod
+    def _axis_name(axis) -> str:
+        names = {0: "x", 1: "y", 2: "z", "x": "x", "y": "y", "z": "z"}
+        if axis not in names:
+            msg = f"Unknown axis: {axis}"
+            raise ValueError(msg)
+        return names[axis]
+
+    def flip(self, axis) -> None:
+        """Mirror the phantom along one axis (0/1/2 or "x"/"y"/"z")."""
+        a = self._axis_name(axis)
+        s = self._shapes.copy()
+        s[a] *= -1
+        for key in ("axis", "a_x", "a_y", "a_z"):
+            s[key][a] *= -1
+        for cas in ("cas1", "cas2", "cas3"):
+            s[cas]["axis"][a] *= -1
+        self._replace(s)
+
+    def translate(self, axis, amount: float) -> None:
+        """Move the phantom by ``amount`` along one axis (clip planes move with it)."""
+        a = self._axis_name(axis)
+        s = self._shapes.copy()
+        s[a] += amount
+        for cas in ("cas1", "cas2", "cas3"):
+            s[cas]["dotprod"] += amount * s[cas]["axis"][a]
+        self._replace(s)
+
+    def multiply_attenuation(self, amount: float) -> None:
+        """Multiply every shape's density by ``amount``."""
+        s = self._shapes.copy()
+        s["rho"] *= amount
+        self._replace(s)
+
+    def superimpose_onto(self, other: FBVolume) -> FBVolume:
+        """``other``'s shapes followed by this phantom's: this one wins where they overlap."""
+        return FBVolume(np.concatenate((other.shapes, self._shapes)))
+
     def voxelise(self, volume: Volume, supersample: int = 1) -> Volume:
         """Add the point-sampled phantom into ``volume`` (add_FORBILD_sampling)."""
         chunk = volume.chunk_spec()

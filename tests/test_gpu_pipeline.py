@@ -81,8 +81,8 @@ def test_forbild_head_phantom_file_matches_oracle(device):
     # (cones and clip planes included), deep overlaps along most rays
     from aprict_amd.volume import pha
 
-    shapes = pha.convert_ph_file_to_shapes(pathlib.Path(__file__).parent / "golden" / "forbild_head" /
-                                           "HeadPhantom.pha")
+    shapes = pha.convert_ph_file_to_shapes(pathlib.Path(pha.__file__).resolve().parents[1] / "data" / "forbild" /
+                                           "head" / "HeadPhantom.pha")
     pg = ph.circular_geometry(16, 64, 64, 0.9, R=500.0, L=1000.0)
     cfg = cbfp.ConfigProject(ssu=2, ssv=2)
     ps = cbfp.ProjectionSet(pg)

@@ -1,5 +1,5 @@
 """FORBILD .pha parsing (CPU): values the format fixes, checked by hand on the
-published head and thorax phantoms (fixtures under tests/golden/), and the
+published head and thorax phantoms (the package's data files), and the
 behaviour of the preprocessor and the arithmetic evaluator."""
 
 from __future__ import annotations
@@ -12,9 +12,9 @@ import pytest
 
 from aprict_amd.volume import forbild, pha
 
-GOLDEN = pathlib.Path(__file__).parent / "golden"
-HEAD = GOLDEN / "forbild_head" / "HeadPhantom.pha"
-THORAX = GOLDEN / "forbild_thorax" / "ThoraxPhantom.pha"
+DATA = pathlib.Path(pha.__file__).resolve().parents[1] / "data" / "forbild"
+HEAD = DATA / "head" / "HeadPhantom.pha"
+THORAX = DATA / "thorax" / "ThoraxPhantom.pha"
 D = math.pi / 180
 
 
