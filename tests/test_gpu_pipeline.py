@@ -94,6 +94,21 @@ def test_forbild_head_phantom_file_matches_oracle(device):
     assert err <= 1e-5
 
 
+def test_forbild_tapered_lung_tree_matches_oracle(device):
+    # the generated lung phantom with tapered tubes: 124 shapes, 60 of them
+    # cones along arbitrary axes, nested outer/inner tubes (mm)
+    shapes = ph.FORBILD.LungPhantom(generation=4, taper=True, fix=True).shapes
+    pg = ph.circular_geometry(12, 64, 64, 8.0, R=500.0, L=1000.0)
+    ps = cbfp.ProjectionSet(pg)
+    cbfp.project(cbfp.ConfigProject(), FBVolume(shapes), ps, block=True)
+    want = orc.project_forbild(cbfp.ConfigProject().contents_fb, pg.to_proj_spec_array(), shapes,
+                               np.zeros((12, 64, 64)))
+    err = rel_l2(ps.data.cpu().numpy(), want)
+    print(f"tapered lung tree ({len(shapes)} primitives) rel L2 {err:.3e}")
+    assert np.abs(want).max() > 10.0
+    assert err <= 1e-5
+
+
 def test_forbild_many_overlaps_use_exact_fallback(device):
     # 60 nested spheres: every central ray meets more shapes than the
     # per-ray interval slots, exercising the recomputing sweep
