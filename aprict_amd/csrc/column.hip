@@ -22,7 +22,7 @@ constexpr int kXcdC = 8;
 constexpr int kViewGroupC = 8;
 // detector rows per forward workgroup (AP_TUNE_PROJECT_COLUMN variants);
 // waves always hold 64 consecutive rows, so every variant gives identical sums
-constexpr int kColRows[3] = {256, 128, 64};
+constexpr int kColRows[3] = {128, 256, 64};
 
 struct SinoCfgC {
     uint32_t V, nu, nv;

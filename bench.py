@@ -74,12 +74,27 @@ def sirt_seconds_per_iter(iters: int = 20) -> dict:
     from aprict_amd.iterators import SIRT_flatpanel
     from aprict_amd.volume import Volume
 
+    from aprict_amd import _abi
+    from aprict_amd.autotune import Autotuner
+
     c = ph.CONFIGS[2]
     pg = ph.circular_geometry(c["views"], c["nu"], c["nv"], c["pitch"])
     truth = make_volume(c["N"], c["voxwid"], c["n_ellipsoids"], c["seed"], "f32")
     P = Projector(pg, vol_shape=truth.shape, voxwid=c["voxwid"])
     b = P.A(truth.data)
     vol = Volume.zeros(truth.shape, voxwid=c["voxwid"])
+    # launch shapes of the forward and the backprojector picked by the autotuner
+    # on scratch buffers of this problem
+    tuner = Autotuner(repeats=2)
+    scratch_p = torch.zeros_like(b)
+    scratch_v = Volume.zeros(truth.shape, voxwid=c["voxwid"])
+    ps_b = cbfp.ProjectionSet(pg, data=b)
+    cfg_bp = cbfp.ConfigBackproject(fast_approx=True, cyl_mask_rad=c["N"] * c["voxwid"] / 2,
+                                    soften_det_edges_proportion=0.05)
+    key = ("sirt", c["N"], c["views"], c["nu"], c["nv"])
+    tuner.tune(_abi.TUNE_PROJECT_COLUMN, key, lambda: P.A(truth.data, out=scratch_p))
+    tuner.tune(_abi.TUNE_BACKPROJECT, key, lambda: cbfp.backproject(cfg_bp, scratch_v, ps_b))
+    del scratch_p, scratch_v
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sirt = SIRT_flatpanel(SIRT_flatpanel.Config(), vol, cbfp.ProjectionSet(pg, data=b))
@@ -93,8 +108,12 @@ def sirt_seconds_per_iter(iters: int = 20) -> dict:
     torch.cuda.synchronize()
     per = (time.perf_counter() - t0) / (iters - 1)
     r = float(torch.linalg.vector_norm(sirt.residual.double()) / torch.linalg.vector_norm(b.double()))
+    tuned = {name: {"variant": tuner.best(op, key), "times_ms": tuner.times_ms(op, key)}
+             for name, op in (("project_column", _abi.TUNE_PROJECT_COLUMN), ("backproject", _abi.TUNE_BACKPROJECT))}
+    for op in (_abi.TUNE_PROJECT_COLUMN, _abi.TUNE_BACKPROJECT):
+        tuner.select(op, None)   # back to the defaults
     return {"workload": "256^3 fp32, 360 views, 384x384, 30 ellipsoids, x0=0", "s_per_iter": per,
-            "setup_s": t_setup, "iterations": iters, "relative_residual_last": r}
+            "setup_s": t_setup, "iterations": iters, "relative_residual_last": r, "tuned": tuned}
 
 
 def storage_leg(P, x32, y32, g32, t_fp32: float, steps: int = 3) -> dict:
@@ -269,6 +288,15 @@ def main() -> int:
     y = torch.zeros(P.sino_shape, dtype=torch.float32, device="cuda")
     g = torch.zeros((n, n, n), dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
+    # the forward's launch shape picked by the autotuner (the adjoint has one)
+    from aprict_amd import _abi
+    from aprict_amd.autotune import Autotuner
+
+    tuner = Autotuner(repeats=2)
+    tkey = ("step", n, V, nu, nv)
+    tuner.tune(_abi.TUNE_PROJECT_COLUMN, tkey, lambda: P.A(x, out=y))
+    tuned = {"project_column": {"variant": tuner.best(_abi.TUNE_PROJECT_COLUMN, tkey),
+                                "times_ms": tuner.times_ms(_abi.TUNE_PROJECT_COLUMN, tkey)}}
 
     def step(record):
         y.zero_()
@@ -328,6 +356,7 @@ def main() -> int:
         "gups_forward": voxel_views / t_a / 1e9,
         "gups_adjoint": voxel_views / t_at / 1e9,
         "dot_test_rel": dot_rel,
+        "tuned": tuned,
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "model": "B_op = V (N^3 s_v + nu nv s_p) bytes per operator application; time = the "

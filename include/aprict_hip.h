@@ -305,7 +305,7 @@ int32_t ap_sirt_iterate(const ap_config_cb_backproject* bp_cfg,
  *                           workgroup 64x4, 64x2, 64x1 (gathers per batch
  *                           4, 4, 8)
  *   AP_TUNE_PROJECT_COLUMN  column forward (upright detectors): detector rows
- *                           per workgroup 256, 128, 64
+ *                           per workgroup 128 (default), 256, 64
  *   AP_TUNE_BACKPROJECT     voxel-driven backprojector: voxels per workgroup
  *                           64x4, 64x2, 64x1                                 */
 #define AP_TUNE_PROJECT_RAY     0
