@@ -28,6 +28,7 @@ struct SinoCfgC {
     uint32_t V, nu, nv;
     int64_t s0, s1, s2;
     uint32_t vec16;    // f32 or f16, v contiguous, rows 16-byte aligned, nv a multiple of 16 bytes
+    float unit;        // adjoint fixed-point unit 2^b (adjoint_fixed_point_unit)
 };
 
 struct ColMap {
@@ -317,8 +318,9 @@ __global__ __launch_bounds__(kRowsF) void k_project_columns(
 // on the order of the adds, so the result is deterministic.  The fixed-point
 // scale S is set per workgroup from the largest |y| over the brick's detector
 // footprints, so that every term rint(w * y * scale * sec * S) fits in 31 bits
-// and the 64-bit sums cannot overflow (views x rays per view < 2^33, checked
-// by the host); the quantisation error is about 1e-9 of the largest term.
+// and the 64-bit sums cannot overflow (adjoint_fixed_point_unit lowers the
+// unit below 2^29 for more than 2^33 views x rays per view); the quantisation
+// error is about 1e-9 of the largest term.
 //
 // Each wave processes its own views, one after another, without workgroup
 // barriers: it projects the brick's corners to find the detector footprint,
@@ -522,8 +524,8 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
         }
         return;
     }
-    // |w * sec| <= sqrt(3) < 2, so every term is below 2^30 in magnitude
-    const float S = 0x1p29f / amax;
+    // |w * sec| <= sqrt(3) < 2, so every term is below 2 * unit in magnitude
+    const float S = sc.unit / amax;
     const float scaleS = scale * S;
     for (uint32_t p = wave; p < sc.V; p += kAdjWaves) {
         if (!view_is_upright(specs, p)) continue;   // wave-uniform
@@ -758,7 +760,7 @@ int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32
     const bool bytes32 = !idx64 && (uint64_t)(array3_max_offset(vol_desc) + 1) * esz <= 0xFFFFFFFFull;
     const size_t shm = ((cap * sizeof(float) + 15) & ~size_t(15)) + cap * (bytes32 ? 4 : 8);
     if (shm > 60 * 1024) return AP_ERR_UNSUPPORTED;
-    SinoCfgC sc{V, nu, nv, s0, s1, s2, 0u};
+    SinoCfgC sc{V, nu, nv, s0, s1, s2, 0u, 0.f};
     const int var = ap_tune_variant(AP_TUNE_PROJECT_COLUMN);
     const uint32_t rows = (uint32_t)kColRows[var];
     ColMap cm;
@@ -815,8 +817,8 @@ int32_t launch_backproject_columns(const RayCfg& rc, uint32_t V, uint32_t nu, ui
         sc.vec16 = kv > 0 && s2 == 1 && nv % kv == 0 && s1 % kv == 0 && s0 % kv == 0 &&
                    ((uintptr_t)sino & 15) == 0;
     }
-    // the 64-bit fixed-point sums stay exact while views x rays per view < 2^33
-    if ((double)V * nu * nv * ssu * ssv >= 8589934592.0) return AP_ERR_UNSUPPORTED;
+    sc.unit = adjoint_fixed_point_unit((double)V * nu * nv * ssu * ssv);
+    if (sc.unit == 0.f) return ap_fail(AP_ERR_INVALID, "too many views x rays per view for the adjoint");
     const uint32_t nbx = (uint32_t)(rc.hi[0] - rc.lo[0] + kTX - 1) / kTX;
     const uint32_t nby = (uint32_t)(rc.hi[1] - rc.lo[1] + kTY - 1) / kTY;
     const uint32_t nbz = (uint32_t)(rc.hi[2] - rc.lo[2] + kTZ - 1) / kTZ;

@@ -15,8 +15,10 @@
 // views.  Per view it visits the detector pixels whose rays can meet the
 // brick (bounding box of the projected brick corners), clips each ray to the
 // brick with the same crossing parameters as the forward walk and scatters
-// weight*y with LDS float atomics; the brick is written to HBM once.  No
-// global atomics, one write per voxel.
+// weight*y into 64-bit fixed-point LDS sums (a first pass over the footprints
+// finds max|y| for the scale); the brick is written to HBM once.  No global
+// atomics, one write per voxel, and the result does not depend on the order
+// of the adds.
 #include "siddon.h"
 
 namespace {
@@ -157,9 +159,15 @@ __global__ __launch_bounds__(kAdjThreads) void k_backproject_matched(
     RayCfg rc, SinoCfg sc, float scale, uint32_t ssu, uint32_t ssv,
     const ap_proj_spec* __restrict__ specs, Tv* __restrict__ vol, IdxT vsx, IdxT vsy, IdxT vsz,
     const Tm* __restrict__ mult, IdxT msx, IdxT msy, IdxT msz, const Tp* __restrict__ sino,
-    uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t per_xcd, int skip_upright) {
-    __shared__ float acc[kBX * kBY * kBZ];
+    uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t per_xcd, int skip_upright, float unit) {
+    // 64-bit fixed point: terms rint(w * y * scale * S) with S = unit /
+    // max|y * scale| over the brick's footprints (|w| <= sqrt(3) voxels, so a
+    // term stays below 2 * unit, and adjoint_fixed_point_unit sizes the unit
+    // so that no sum overflows); integer sums do not depend on the order of
+    // the adds, so the result is deterministic.
+    __shared__ unsigned long long acc[kBX * kBY * kBZ];
     __shared__ BrickView bv[kViewBatchAdj];
+    __shared__ float wred[kAdjThreads / 64];
     const uint32_t L = blockIdx.x;
     const uint32_t logical = (L % kXcd) * per_xcd + L / kXcd;
     const uint32_t nbr = nbx * nby * nbz;
@@ -169,12 +177,11 @@ __global__ __launch_bounds__(kAdjThreads) void k_backproject_matched(
     int blo[3] = {rc.lo[0] + (int)bi * kBX, rc.lo[1] + (int)bj * kBY, rc.lo[2] + (int)bk * kBZ};
     int bhi[3] = {min(blo[0] + kBX, rc.hi[0]), min(blo[1] + kBY, rc.hi[1]), min(blo[2] + kBZ, rc.hi[2])};
     const int tid = threadIdx.x;
-    for (int e = tid; e < kBX * kBY * kBZ; e += kAdjThreads) acc[e] = 0.f;
+    for (int e = tid; e < kBX * kBY * kBZ; e += kAdjThreads) acc[e] = 0ull;
     const int maxit = kBX + kBY + kBZ + 3;
     const float half_u = 0.5f, half_v = 0.5f;
-    for (uint32_t p0 = 0; p0 < sc.V; p0 += kViewBatchAdj) {
-        const uint32_t nb = min((uint32_t)kViewBatchAdj, sc.V - p0);
-        __syncthreads();
+    // footprints of views [p0, p0 + nb) into bv (one thread per view)
+    auto fill_views = [&](uint32_t p0, uint32_t nb) {
         if (tid < (int)nb && skip_upright && view_is_upright(specs, p0 + tid)) {
             bv[tid].j0 = 0; bv[tid].j1 = -1; bv[tid].k0 = 0; bv[tid].k1 = -1;
         } else if (tid < (int)nb) {
@@ -203,6 +210,50 @@ __global__ __launch_bounds__(kAdjThreads) void k_backproject_matched(
                 B.j0 = 0; B.j1 = (int)sc.nu - 1; B.k0 = 0; B.k1 = (int)sc.nv - 1;
             }
         }
+    };
+    // pass 1: largest |y| over the footprints sets the fixed-point scale
+    float ymax = 0.f;
+    for (uint32_t p0 = 0; p0 < sc.V; p0 += kViewBatchAdj) {
+        const uint32_t nb = min((uint32_t)kViewBatchAdj, sc.V - p0);
+        __syncthreads();
+        fill_views(p0, nb);
+        __syncthreads();
+        for (uint32_t q = 0; q < nb; q++) {
+            const BrickView& B = bv[q];
+            if (B.j1 < B.j0 || B.k1 < B.k0) continue;
+            const int nk = B.k1 - B.k0 + 1;
+            const int npx = (B.j1 - B.j0 + 1) * nk;
+            const uint32_t p = p0 + q;
+            for (int pi = tid; pi < npx; pi += kAdjThreads) {
+                const int j = B.j0 + pi / nk, k = B.k0 + pi % nk;
+                const float y = to_f32(sino[(int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2]);
+                ymax = fmaxf(ymax, fabsf(y));
+                if (y != y) ymax = __builtin_inff();
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) ymax = fmaxf(ymax, __shfl_xor(ymax, o, 64));
+    if ((tid & 63) == 0) wred[tid >> 6] = ymax;
+    __syncthreads();
+    ymax = 0.f;
+    for (int w = 0; w < kAdjThreads / 64; w++) ymax = fmaxf(ymax, wred[w]);
+    const float amax = ymax * fabsf(scale);
+    if (amax == 0.f) return;   // block-uniform: nothing to add
+    if (!(amax < __builtin_inff())) {
+        // non-finite data: the sum is not representable; propagate NaN
+        for (int e = tid; e < kBX * kBY * kBZ; e += kAdjThreads) {
+            const int li = e / (kBY * kBZ), lj = (e / kBZ) % kBY, lk = e % kBZ;
+            const int gi = blo[0] + li, gj = blo[1] + lj, gk = blo[2] + lk;
+            if (gi >= bhi[0] || gj >= bhi[1] || gk >= bhi[2]) continue;
+            vol[(IdxT)gi * vsx + (IdxT)gj * vsy + (IdxT)gk * vsz] = from_f32<Tv>(__builtin_nanf(""));
+        }
+        return;
+    }
+    const float S = unit / amax;
+    for (uint32_t p0 = 0; p0 < sc.V; p0 += kViewBatchAdj) {
+        const uint32_t nb = min((uint32_t)kViewBatchAdj, sc.V - p0);
+        __syncthreads();
+        fill_views(p0, nb);
         __syncthreads();
         for (uint32_t q = 0; q < nb; q++) {
             const BrickView& B = bv[q];
@@ -214,7 +265,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_backproject_matched(
                 const int j = B.j0 + pi / nk, k = B.k0 + pi % nk;
                 const float y = to_f32(sino[(int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2]);
                 if (y == 0.f) continue;
-                const float ys = y * scale;
+                const float ys = y * scale * S;
                 for (uint32_t su = 0; su < ssu; su++)
                     for (uint32_t sv = 0; sv < ssv; sv++) {
                         Ray R;
@@ -232,7 +283,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_backproject_matched(
                             float w;
                             int o;
                             const bool more = W.step(w, o);
-                            atomicAdd(&acc[o], w * ys);
+                            atomicAdd(&acc[o], (unsigned long long)(long long)(int)__builtin_rintf(w * ys));
                             if (!more) break;
                         }
                     }
@@ -240,15 +291,16 @@ __global__ __launch_bounds__(kAdjThreads) void k_backproject_matched(
         }
     }
     __syncthreads();
+    const double inv_s = 1.0 / (double)S;
     for (int e = tid; e < kBX * kBY * kBZ; e += kAdjThreads) {
         const int li = e / (kBY * kBZ), lj = (e / kBZ) % kBY, lk = e % kBZ;
         const int gi = blo[0] + li, gj = blo[1] + lj, gk = blo[2] + lk;
         if (gi >= bhi[0] || gj >= bhi[1] || gk >= bhi[2]) continue;
-        const float a = acc[e];
-        if (a == 0.f) continue;
+        const long long a = (long long)acc[e];
+        if (a == 0) continue;
         const IdxT ov = (IdxT)gi * vsx + (IdxT)gj * vsy + (IdxT)gk * vsz;
         const float m = mult ? to_f32(mult[(IdxT)gi * msx + (IdxT)gj * msy + (IdxT)gk * msz]) : 1.f;
-        vol[ov] = from_f32<Tv>(__builtin_fmaf(m, a, to_f32(vol[ov])));
+        vol[ov] = from_f32<Tv>(__builtin_fmaf(m, (float)((double)a * inv_s), to_f32(vol[ov])));
     }
 }
 
@@ -416,6 +468,8 @@ extern "C" int32_t ap_backproject_matched(const ap_config_cb_project* cfg,
                                     specs, vol_desc, vol, mdt, ms, multiplier, sino_desc->dtype, sino, idx64, s);
     if (st != AP_OK && st != AP_ERR_UNSUPPORTED) return st;
     const int skip = st == AP_OK ? 1 : 0;
+    const float unit = adjoint_fixed_point_unit((double)sc.V * sc.nu * sc.nv * cfg->ssu * cfg->ssv);
+    AP_REQUIRE(unit > 0.f, "too many views x rays per view for the adjoint");
     return dispatch_dtype(vol_desc->dtype, [&](auto tv) -> int32_t {
         using Tv = decltype(tv);
         return dispatch_dtype(sino_desc->dtype, [&](auto tp) -> int32_t {
@@ -431,14 +485,14 @@ extern "C" int32_t ap_backproject_matched(const ap_config_cb_project* cfg,
                                            0, s, rc, sc, scale, cfg->ssu, cfg->ssv, specs, (Tv*)vol,
                                            vol_desc->strides[0], vol_desc->strides[1], vol_desc->strides[2],
                                            (const Tm*)multiplier, ms[0], ms[1], ms[2], (const Tp*)sino, nbx,
-                                           nby, nbz, per_xcd, skip);
+                                           nby, nbz, per_xcd, skip, unit);
                     } else {
                         hipLaunchKernelGGL((k_backproject_matched<Tv, Tp, Tm, int32_t>), grid, dim3(kAdjThreads),
                                            0, s, rc, sc, scale, cfg->ssu, cfg->ssv, specs, (Tv*)vol,
                                            (int32_t)vol_desc->strides[0], (int32_t)vol_desc->strides[1],
                                            (int32_t)vol_desc->strides[2], (const Tm*)multiplier,
                                            (int32_t)ms[0], (int32_t)ms[1], (int32_t)ms[2], (const Tp*)sino,
-                                           nbx, nby, nbz, per_xcd, skip);
+                                           nbx, nby, nbz, per_xcd, skip, unit);
                     }
                     return ap_check_launch("backproject_matched");
                 }
