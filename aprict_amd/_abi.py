@@ -25,7 +25,8 @@ AP_ERR_UNSUPPORTED = -2
 AP_ERR_HIP = -3
 
 AP_F32, AP_F16, AP_BF16, AP_F64 = 0, 1, 2, 3
-AP_RAMP_RAMLAK, AP_RAMP_SINC = 0, 1
+AP_RAMP_RAMLAK, AP_RAMP_SINC, AP_RAMP_HILBERT, AP_RAMP_HILBERT_HALF = 0, 1, 2, 3
+AP_FBP_GBC, AP_FBP_FAN, AP_FBP_FDK = 0, 1, 2
 
 
 # --------------------------------------------------------------------------
@@ -142,12 +143,15 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ap_version": (_I32, []),
     "ap_struct_size": (_I32, [_I32]),
     "ap_project_voxel": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _P, _P]),
-    "ap_backproject": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _P]),
+    "ap_backproject": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _A3, _P, _P]),
     "ap_backproject_matched": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _F32, _P]),
     "ap_project_forbild": (_I32, [_P, _P, _U32, _P, _U32, _A3, _P, _P, _P]),
     "ap_voxelise_forbild": (_I32, [_P, _U32, _P, _A3, _P, _U32, _P]),
     "ap_forbild_bboxes": (_I32, [_P, _U32, _P, _P]),
     "ap_ramp_filter_rows": (_I32, [_A3, _P, _A3, _P, _F64, _I32, _I32, _P]),
+    "ap_fbp_rotate_weight": (_I32, [_A3, _P, _A3, _P, _P, _F64, _F64, _I32, _P]),
+    "ap_fbp_unrotate": (_I32, [_A3, _P, _A3, _P, _P, _P]),
+    "ap_gbc_weight_integ": (_I32, [_P, _U32, _P, _U32, _U32, _P, _P]),
     "ap_array_fill": (_I32, [_A3, _P, _F64, _P]),
     "ap_array_scale": (_I32, [_A3, _P, _F64, _P]),
     "ap_array_axpby": (_I32, [_A3, _P, _F64, _A3, _P, _F64, _P]),

@@ -503,9 +503,17 @@ def backproject(config: ConfigBackproject, volume: Volume, projs: ProjectionSet,
     if bool(cfg["apply_summand_multiplier"]) != (summand_multiplier is not None):
         msg = "To prevent unintended behaviour, the config 'apply_summand_multiplier' must be true if and only if a summand_multiplier is included."
         raise ValueError(msg)
-    if int(cfg["backproj_weighting_preset"]) != 0 or secondary_volume is not None:
-        msg = "GBC weighting presets (backproj_weighting_preset != 0) are not supported"
-        raise NotImplementedError(msg)
+    sdat = None
+    if int(cfg["backproj_weighting_preset"]) == 7:
+        # the GBC kernel also accumulates its weights (CB_backproject.cu:1033-1108)
+        if secondary_volume is None:
+            msg = "backproj_weighting_preset 7 (GBC) requires a secondary_volume for the sum of the weights."
+            raise ValueError(msg)
+        if secondary_volume.shape != volume.shape or secondary_volume.dtype != volume.dtype:
+            msg = "The secondary volume must have the volume's shape and dtype."
+            raise ValueError(msg)
+        sdat = secondary_volume.data
+        _abi.require_device(sdat, "secondary volume")
     data = projs._flat_data()
     vdat = volume.data
     _abi.require_device(data, "projection data")
@@ -514,9 +522,10 @@ def backproject(config: ConfigBackproject, volume: Volume, projs: ProjectionSet,
     chunk = _chunk_of(volume)
     mdat = summand_multiplier.data if summand_multiplier is not None else None
     md = _abi.array3(mdat) if mdat is not None else None
+    sd = _abi.array3(sdat) if sdat is not None else None
     _abi.check(lib.ap_backproject(_abi.struct_ptr(cfg), _abi.ptr(projs.specs_device()), data.shape[0],
                                   _abi.struct_ptr(chunk), _abi.array3(vdat), _abi.ptr(vdat), md,
-                                  _abi.ptr(mdat), _abi.array3(data), _abi.ptr(data),
+                                  _abi.ptr(mdat), _abi.array3(data), _abi.ptr(data), sd, _abi.ptr(sdat),
                                   _abi.stream_handle(data.device)), "backproject")
     if block:
         torch.cuda.synchronize(data.device)

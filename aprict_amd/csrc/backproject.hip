@@ -2,7 +2,9 @@
 // semantics (ap_backproject): the exact intersection-length branch and the
 // fast interpolating (FDK / SIRT) branch of CB_backproject.cu:313-733,
 // including sub-voxel sampling, cylinder mask, detector-edge softening,
-// Z-power weights and the summand multiplier (:952-1027).
+// Z-power weights, the weighting presets of the direct ray-casting branch
+// and the summand multiplier (:952-1027); and the GBC backprojector for a
+// cylindrical source locus with its weight volume (:736-1108).
 //
 // One thread per voxel; a wavefront covers 64 consecutive voxels along the
 // volume's z axis.  Views are processed in batches of kViewBatch: the block
@@ -17,6 +19,7 @@ constexpr int kBz = 64;
 constexpr int kBpRowsY[3] = {4, 2, 1};
 constexpr int kViewBatch = 64;
 static_assert(kBz * kBpRowsY[2] >= kViewBatch, "every variant stages a full view batch");
+constexpr int kGbcRows = 2;   // voxel rows per workgroup of the GBC kernel
 
 // Per-view constants (FP32) of the central projection x -> detector:
 //   cu = (Au.x + au)/(W.x + w0),  cv = (Av.x + av)/(W.x + w0)  (pixels from
@@ -70,6 +73,8 @@ __device__ void make_view_map(const ap_proj_spec* specs, uint32_t p, ViewMap& m)
 
 struct BpCfg {
     bool exact, all1s, mask, direct, bilinear, use_mult;
+    int preset;           // backproj_weighting_preset
+    float sv_a, sv_b, sh_a, sh_b;   // GBC softening: vartheta_v, its span, vartheta_h, its span
     float mask_r2, sep, zval;
     float inv_sep;        // 1 / sep (0 when sep == 0)
     float w3;             // voxwid^3
@@ -135,6 +140,48 @@ __device__ __forceinline__ float soften(const BpCfg& c, const ViewMap& m, float 
     return val;
 }
 
+// Weight of the direct ray-casting branch for backproj_weighting_preset 1-4
+// and 6 (CB_backproject.cu:528-600), applied to `mult`.  R is Z_on_val_factor_val, the
+// source locus is a cylinder about the z axis and zz the distance from the
+// source to the voxel along the source->detector-centre axis.  Other presets
+// add no weight there (preset 7 has its own kernel).
+__device__ __forceinline__ float apply_preset(const BpCfg& c, const ViewMap& m, const float x[3], float zz,
+                                              float mult) {
+    const float R = c.zval;
+    const float r0 = x[0] - m.S[0], r1 = x[1] - m.S[1];
+    const float ell = sqrtf(r0 * r0 + r1 * r1) / R;   // horizontal source->voxel distance / R
+    const float zr = zz / R;
+    const float cr = fmaxf(0.f, fminf(1.f, fabsf(zr / ell)));
+    const float dr = cr - ell;
+    // voxel position relative to the source direction, in units of R^2
+    const float vy = (m.S[0] * x[1] - m.S[1] * x[0]) / R / R;
+    const float vx = (m.S[0] * x[0] + m.S[1] * x[1]) / R / R;
+    switch (c.preset) {
+    case 1:
+        return mult / (cr * cr * (cr * cr + dr * dr));
+    case 2: {
+        const float q = dr / cr;
+        const float f = (1.f - q * q) / (1.f + q * q);
+        return mult * (-2.f * (f * f) * (-vy) * (1.f + vx / zr) / (ell * ell));
+    }
+    case 3:
+        return mult / (zr * cr * cr);
+    case 4:
+        return mult * ((vy / zr) / zr);
+    case 6: {
+        const float q = dr / cr;
+        const float f = (1.f - q * q) / (1.f + q * q);
+        return mult * (-2.f * (f * f) / (1.f - q) * (-vy) * (1.f + vx / zr) / zr);
+    }
+    default:
+        return mult;
+    }
+}
+
+template <typename Tp>
+__device__ __forceinline__ float sample_det(const BpCfg& c, const ViewMap& m, const Tp* __restrict__ img,
+                                            float cu, float cv);
+
 // The interpolating branch (CB_backproject.cu:478-691): the voxel centre's
 // projection, weighted by the FDK magnification factor (or 1 with direct
 // ray-casting) and Z powers, read bilinearly or at the nearest pixel.  The
@@ -148,12 +195,22 @@ __device__ __forceinline__ float bp_fast_view(const BpCfg& c, const ViewMap& m, 
     const float cu = (m.Au[0] * x[0] + m.Au[1] * x[1] + m.Au[2] * x[2] + m.au) * iw;
     const float cv = (m.Av[0] * x[0] + m.Av[1] * x[1] + m.Av[2] * x[2] + m.av) * iw;
     float mult = c.direct ? 1.f : m.kq * c.w3 * (iw * iw);
-    if (c.zl != 0 || c.zv != 0) {
+    const bool pre = c.direct && c.preset != 0;
+    if (c.zl != 0 || c.zv != 0 || pre) {
         const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
         if (c.zl != 0) mult *= powi(zz / m.Lnorm, c.zl);
         if (c.zv != 0) mult *= powi(zz / c.zval, c.zv);
+        if (pre) mult = apply_preset(c, m, x, zz, mult);
     }
     if (mult == 0.f) return 0.f;
+    return soften(c, m, cu, cv, sample_det(c, m, img, cu, cv) * mult);
+}
+
+// Detector value at (cu, cv) pixels from the centre: bilinear
+// (CB_backproject.cu:221-254) or nearest (:256-271), zero off the detector.
+template <typename Tp>
+__device__ __forceinline__ float sample_det(const BpCfg& c, const ViewMap& m, const Tp* __restrict__ img,
+                                            float cu, float cv) {
     auto rd = [&](int j, int k) -> float {
         if (c.off32) return to_f32(img[j * (int32_t)c.s1 + k * (int32_t)c.s2]);
         return to_f32(img[(int64_t)j * c.s1 + (int64_t)k * c.s2]);
@@ -181,7 +238,7 @@ __device__ __forceinline__ float bp_fast_view(const BpCfg& c, const ViewMap& m, 
         const int j = (int)floorf(fj + 0.5f), k = (int)floorf(fk + 0.5f);
         if (j >= 0 && k >= 0 && j < (int)c.nu && k < (int)c.nv) y = rd(j, k);
     }
-    return soften(c, m, cu, cv, y * mult);
+    return y;
 }
 
 template <typename Tp>
@@ -305,14 +362,196 @@ __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
     vol[o] = from_f32<Tv>(to_f32(vol[o]) + acc1 * mval);
 }
 
+// cos^2-type softening window s(a, b; c) of the GBC paper's appendix
+// (CB_backproject.cu:274-311): 1 for |c| < a - b, 0 for |c| > a, and
+// 3x^2 - 2x^3 in between with x = (sin|c| - sin a) / (sin(a - b) - sin a).
+__device__ __forceinline__ float cossq_soft(float a, float b, float c) {
+    const float ac = fabsf(c);
+    if (ac > a) return 0.f;
+    if (ac < a - b) return 1.f;
+    const float g = 1.f / (sinf(a - b) - sinf(a));
+    const float x = g * sinf(ac) - g * sinf(a);
+    return fmaxf(0.f, fminf(1.f, 3.f * x * x - 2.f * x * x * x));
+}
+
+// GBC weight of one view for a source on a cylinder of radius R about the z
+// axis (CB_backproject.cu:736-790, eq. (23) of the GBC paper): with ell the
+// horizontal source->voxel distance, c the |cosine| of the horizontal angle
+// between that direction and the direction to the axis (both over R) and
+// theta the elevation of the voxel seen from the source,
+//   w = cos^3(theta) c / (c^2 + (c - ell)^2) * s(v; theta) * s(h; acos c).
+struct GbcSoft {
+    float R;                   // source radius (the cylinder's axis is the z axis)
+    float va, vb, ha, hb;      // soft_vartheta_v, its span, soft_vartheta_h, its span
+};
+
+__device__ __forceinline__ float gbc_weight(const GbcSoft& g, const float S[3], const float x[3]) {
+    const float R = g.R;
+    const float lx = -S[0], ly = -S[1];
+    const float r0 = x[0] - S[0], r1 = x[1] - S[1], r2 = x[2] - S[2];
+    const float zz = (r0 * lx + r1 * ly) / sqrtf(lx * lx + ly * ly);
+    const float ell = sqrtf(r0 * r0 + r1 * r1) / R;
+    const float zr = zz / R;
+    const float cr = fmaxf(0.f, fminf(1.f, fabsf(zr / ell)));
+    const float dr = cr - ell;
+    const float th = atan2f(r2 / R, ell);
+    const float ce = cosf(th);
+    const float w = ce * ce * ce * cr / fmaxf(1e-9f, cr * cr + dr * dr);
+    return w * (cossq_soft(g.va, g.vb, th) * cossq_soft(g.ha, g.hb, acosf(cr)));
+}
+
+// one view's GBC contribution: value += w * detector(x), weight += w
+template <typename Tp>
+__device__ __forceinline__ void gbc_view(const BpCfg& c, const ViewMap& m, const Tp* __restrict__ img,
+                                         const float x[3], float& val, float& wsum) {
+    const float w = gbc_weight(GbcSoft{c.zval, c.sv_a, c.sv_b, c.sh_a, c.sh_b}, m.S, x);
+    wsum += w;
+    if (w == 0.f) return;
+    const float iw = __builtin_amdgcn_rcpf(m.W[0] * x[0] + m.W[1] * x[1] + m.W[2] * x[2] + m.w0);
+    const float cu = (m.Au[0] * x[0] + m.Au[1] * x[1] + m.Au[2] * x[2] + m.au) * iw;
+    const float cv = (m.Av[0] * x[0] + m.Av[1] * x[1] + m.Av[2] * x[2] + m.av) * iw;
+    val += sample_det(c, m, img, cu, cv) * w;
+}
+
+// GBC backprojection (backproject_CB_voltex_GBC_cyl, CB_backproject.cu:1033-1108):
+// the volume receives sum_views w * detector value, the secondary volume the
+// sum of the weights w; sub-voxel means as in the other branches, both times
+// the summand multiplier.  No mask, softening or Z powers apply.
+template <typename Tv, typename Tp, typename Tm>
+__global__ __launch_bounds__(kBz* kGbcRows) void k_backproject_gbc(
+    BpCfg c, const ap_proj_spec* __restrict__ specs, uint32_t num_projs,
+    Tv* __restrict__ vol, int64_t vs0, int64_t vs1, int64_t vs2,
+    Tv* __restrict__ sec, int64_t ss0, int64_t ss1, int64_t ss2,
+    const Tm* __restrict__ mult, int64_t ms0, int64_t ms1, int64_t ms2,
+    const Tp* __restrict__ sino) {
+    __shared__ ViewMap maps[kViewBatch];
+    const uint32_t ck = blockIdx.x * kBz + threadIdx.x;
+    const uint32_t cj = blockIdx.y * kGbcRows + threadIdx.y;
+    const uint32_t ci = blockIdx.z;
+    const bool active = ck < c.shp[2] && cj < c.shp[1] && ci < c.shp[0];
+    const uint32_t gi = ci + c.off[0], gj = cj + c.off[1], gk = ck + c.off[2];
+    const float xc[3] = {c.c000[0] + (float)gi * c.w, c.c000[1] + (float)gj * c.w,
+                         c.c000[2] + (float)gk * c.w};
+    const uint32_t nss = c.ssx * c.ssy * c.ssz;
+    float val = 0.f, wsum = 0.f;
+    const int tid = threadIdx.y * kBz + threadIdx.x;
+    if (nss == 1) {
+        for (uint32_t p0 = 0; p0 < num_projs; p0 += kViewBatch) {
+            const uint32_t nb = min((uint32_t)kViewBatch, num_projs - p0);
+            __syncthreads();
+            if (tid < (int)nb) make_view_map(specs, p0 + tid, maps[tid]);
+            __syncthreads();
+            if (!active) continue;
+            for (uint32_t q = 0; q < nb; q++)
+                gbc_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc, val, wsum);
+        }
+    } else if (active) {
+        // sub-voxel loop order x, y, z (CB_backproject.cu:822-890)
+        float vx = 0.f, wx = 0.f;
+        for (uint32_t si = 0; si < c.ssx; si++) {
+            float vy = 0.f, wy = 0.f;
+            for (uint32_t sj = 0; sj < c.ssy; sj++) {
+                float vz = 0.f, wz = 0.f;
+                for (uint32_t sk = 0; sk < c.ssz; sk++) {
+                    const float xs[3] = {xc[0] + (-0.5f + (0.5f + si) / c.ssx) * c.w,
+                                         xc[1] + (-0.5f + (0.5f + sj) / c.ssy) * c.w,
+                                         xc[2] + (-0.5f + (0.5f + sk) / c.ssz) * c.w};
+                    float a = 0.f, b = 0.f;
+                    for (uint32_t p = 0; p < num_projs; p++) {
+                        ViewMap m;
+                        make_view_map(specs, p, m);
+                        gbc_view(c, m, sino + (int64_t)p * c.s0, xs, a, b);
+                    }
+                    vz += a;
+                    wz += b;
+                }
+                vy += vz / c.ssz;
+                wy += wz / c.ssz;
+            }
+            vx += vy / c.ssy;
+            wx += wy / c.ssy;
+        }
+        val = vx / c.ssx;
+        wsum = wx / c.ssx;
+    }
+    if (!active || (val == 0.f && wsum == 0.f)) return;
+    float mval = 1.f;
+    if (c.use_mult) mval = to_f32(mult[(int64_t)gi * ms0 + (int64_t)gj * ms1 + (int64_t)gk * ms2]);
+    if (val != 0.f) {
+        const int64_t o = (int64_t)gi * vs0 + (int64_t)gj * vs1 + (int64_t)gk * vs2;
+        vol[o] = from_f32<Tv>(to_f32(vol[o]) + val * mval);
+    }
+    if (wsum != 0.f) {
+        const int64_t o = (int64_t)gi * ss0 + (int64_t)gj * ss1 + (int64_t)gk * ss2;
+        sec[o] = from_f32<Tv>(to_f32(sec[o]) + wsum * mval);
+    }
+}
+
+// Integral of the GBC weight over a full cylindrical source locus for
+// points (r, 0, 0), R = 1 (get_GBC_weight_integ, CB_backproject.cu:1111-1168):
+// a workgroup sums kIntegAngles source angles x all heights for one r in
+// FP64 and adds the sum, times the quadrature weight, to out[r].
+constexpr int kIntegThreads = 256, kIntegAngles = 16;
+
+__global__ __launch_bounds__(kIntegThreads) void k_gbc_weight_integ(GbcSoft g, const float* __restrict__ r_on_R,
+                                                                    uint32_t n, double* __restrict__ out,
+                                                                    uint32_t nz, uint32_t nt) {
+    __shared__ double red[kIntegThreads / 64];
+    const uint32_t ri = blockIdx.y;
+    const float x[3] = {r_on_R[ri], 0.f, 0.f};
+    const float maxz = tanf(g.va) * (1.f + r_on_R[n - 1]);
+    const uint32_t a0 = blockIdx.x * kIntegAngles;
+    const uint32_t na = min((uint32_t)kIntegAngles, nt - a0);
+    double acc = 0.0;
+    for (uint32_t e = threadIdx.x; e < na * nz; e += kIntegThreads) {
+        const uint32_t a = a0 + e / nz, b = e % nz;
+        const float t = (float)a / (float)nt * 3.1415926535898f;
+        const float S[3] = {cosf(t), sinf(t), maxz * (float)b / (float)(nz - 1)};
+        const float w = gbc_weight(g, S, x);
+        acc += (double)(b == 0 ? 0.5f * w : w);   // trapezoid end weight at z = 0
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < kIntegThreads / 64; w++) sum += red[w];
+        const float q = (4.f * maxz * 3.1415926535898f) / ((float)nz * (float)nt);
+        atomicAdd(&out[ri], sum * (double)q);
+    }
+}
+
 }  // namespace
+
+extern "C" int32_t ap_gbc_weight_integ(const float* r_on_R, uint32_t n, double* out, uint32_t num_z_samps,
+                                       uint32_t num_ang_samps, const ap_config_cb_backproject* cfg,
+                                       void* stream) {
+    AP_REQUIRE(cfg, "NULL config");
+    if (n == 0) return AP_OK;
+    AP_REQUIRE(r_on_R && out, "NULL r_on_R or out");
+    AP_REQUIRE(num_z_samps >= 2, "num_z_samps must be >= 2");
+    AP_REQUIRE(num_ang_samps >= 1, "num_ang_samps must be >= 1");
+    AP_REQUIRE(n <= 65535u, "at most 65535 radii per call");
+    const GbcSoft g{1.f, cfg->soft_vartheta_v, cfg->soft_vartheta_v_softspan, cfg->soft_vartheta_h,
+                    cfg->soft_vartheta_h_softspan};
+    dim3 grid((num_ang_samps + kIntegAngles - 1) / kIntegAngles, n, 1);
+    hipLaunchKernelGGL(k_gbc_weight_integ, grid, dim3(kIntegThreads), 0, (hipStream_t)stream, g, r_on_R, n, out,
+                       num_z_samps, num_ang_samps);
+    return ap_check_launch("gbc_weight_integ");
+}
 
 int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_proj_spec* specs,
                                  uint32_t num_projs, const ap_vol_tex_chunk_spec* chunk,
                                  const ap_array3* vol_desc, void* vol,
                                  const ap_array3* mult_desc, const void* multiplier,
-                                 const ap_array3* sino_desc, const void* sino, hipStream_t s) {
+                                 const ap_array3* sino_desc, const void* sino,
+                                 const ap_array3* sec_desc, void* secondary, hipStream_t s) {
     BpCfg c;
+    c.preset = cfg->backproj_weighting_preset;
+    c.sv_a = cfg->soft_vartheta_v;
+    c.sv_b = cfg->soft_vartheta_v_softspan;
+    c.sh_a = cfg->soft_vartheta_h;
+    c.sh_b = cfg->soft_vartheta_h_softspan;
     c.exact = cfg->exact_line_integ_backproj != 0;
     c.all1s = cfg->backproject_all_1s != 0;
     c.mask = cfg->mask_to_cylinder != 0;
@@ -354,6 +593,16 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
                               std::is_same<Tm, double>::value || std::is_same<Tp, bf16_t>::value) {
                     return ap_fail(AP_ERR_UNSUPPORTED, "unsupported dtype combination");
                 } else {
+                    if (c.preset == 7) {
+                        dim3 gblock(kBz, kGbcRows, 1);
+                        dim3 ggrid((c.shp[2] + kBz - 1) / kBz, (c.shp[1] + kGbcRows - 1) / kGbcRows, c.shp[0]);
+                        hipLaunchKernelGGL((k_backproject_gbc<Tv, Tp, Tm>), ggrid, gblock, 0, s, c, specs,
+                                           num_projs, (Tv*)vol, vol_desc->strides[0], vol_desc->strides[1],
+                                           vol_desc->strides[2], (Tv*)secondary, sec_desc->strides[0],
+                                           sec_desc->strides[1], sec_desc->strides[2], (const Tm*)multiplier,
+                                           ms[0], ms[1], ms[2], (const Tp*)sino);
+                        return ap_check_launch("backproject_gbc");
+                    }
                     auto launch = [&](auto kern) {
                         hipLaunchKernelGGL(kern, grid, block, 0, s, c, specs, num_projs, (Tv*)vol,
                                            vol_desc->strides[0], vol_desc->strides[1], vol_desc->strides[2],
@@ -373,7 +622,8 @@ extern "C" int32_t ap_backproject(const ap_config_cb_backproject* cfg, const ap_
                                   uint32_t num_projs, const ap_vol_tex_chunk_spec* chunk,
                                   const ap_array3* vol_desc, void* vol,
                                   const ap_array3* mult_desc, const void* multiplier,
-                                  const ap_array3* sino_desc, const void* sino, void* stream) {
+                                  const ap_array3* sino_desc, const void* sino,
+                                  const ap_array3* sec_desc, void* secondary, void* stream) {
     AP_REQUIRE(cfg && chunk && specs, "NULL config, chunk or specs");
     int32_t st = ap_check_array3(vol_desc, vol, "volume", true, true, false);
     if (st) return st;
@@ -381,9 +631,16 @@ extern "C" int32_t ap_backproject(const ap_config_cb_backproject* cfg, const ap_
     if (st) return st;
     AP_REQUIRE(sino_desc->shape[0] == num_projs, "sino has %u views, num_projs=%u",
                sino_desc->shape[0], num_projs);
-    if (cfg->backproj_weighting_preset != 0)
-        return ap_fail(AP_ERR_UNSUPPORTED, "backproj_weighting_preset %u (GBC) is not supported",
-                       (unsigned)cfg->backproj_weighting_preset);
+    if (cfg->backproj_weighting_preset == 7) {
+        // the GBC kernel also writes the sum of its weights (CB_backproject.cu:987, 1072-1108)
+        AP_REQUIRE(secondary != nullptr, "backproj_weighting_preset 7 needs a secondary volume");
+        if ((st = ap_check_array3(sec_desc, secondary, "secondary volume", true, true, false))) return st;
+        AP_REQUIRE(sec_desc->dtype == vol_desc->dtype, "secondary volume dtype != volume dtype");
+        for (int a = 0; a < 3; a++)
+            AP_REQUIRE(sec_desc->shape[a] == vol_desc->shape[a], "secondary volume shape != volume shape");
+        AP_REQUIRE(secondary != vol, "the secondary volume must not be the volume");
+        AP_REQUIRE(cfg->Z_on_val_factor_val != 0.f, "preset 7 needs the source radius in Z_on_val_factor_val");
+    }
     AP_REQUIRE(cfg->ssx >= 1 && cfg->ssy >= 1 && cfg->ssz >= 1, "ssx/ssy/ssz must be >= 1");
     AP_REQUIRE(chunk->voxwid > 0.f, "voxwid must be positive");
     if ((cfg->apply_summand_multiplier != 0) != (multiplier != nullptr))
@@ -401,5 +658,5 @@ extern "C" int32_t ap_backproject(const ap_config_cb_backproject* cfg, const ap_
     if (num_projs == 0 || shp[0] == 0 || shp[1] == 0 || shp[2] == 0) return AP_OK;
     AP_REQUIRE(shp[0] <= 65535u, "chunk x extent above 65535");
     return launch_backproject_voxel(cfg, specs, num_projs, chunk, vol_desc, vol, mult_desc,
-                                    multiplier, sino_desc, sino, (hipStream_t)stream);
+                                    multiplier, sino_desc, sino, sec_desc, secondary, (hipStream_t)stream);
 }

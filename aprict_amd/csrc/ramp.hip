@@ -2,7 +2,7 @@
 // convolution along axis 1 (u) of a [V][nu][nv] sinogram.
 //
 // The reference filters with an FP32 FFT (cb/ramp_filter.py:139-240, taps of
-// ramp_kernels.py:17-59 transformed in FP64); the convolution it implements is
+// ramp_kernels.py:27-94 transformed in FP64); the convolution it implements is
 // the "same"-mode linear convolution with those taps, which this kernel
 // evaluates directly (no padding, no complex arithmetic, FP32 data with the
 // taps rounded once from FP64).  The FDK cosine pre-weight of
@@ -32,6 +32,8 @@ __global__ __launch_bounds__(kLanes* kRows) void k_ramp_rows(
         const long d = (long)n - (long)(nu - 1);
         double t;
         if (kernel == AP_RAMP_SINC) t = 8.0 / (1.0 - 4.0 * (double)d * (double)d) / p;
+        else if (kernel == AP_RAMP_HILBERT) t = d == 0 ? 0.0 : 2.0 / (double)d;   // independent of p
+        else if (kernel == AP_RAMP_HILBERT_HALF) t = 2.0 / ((double)d + 0.5);
         else if (d == 0) t = pi * pi / p;
         else if (d % 2 == 0) t = 0.0;
         else t = -4.0 / ((double)d * (double)d) / p;
@@ -85,7 +87,9 @@ extern "C" int32_t ap_ramp_filter_rows(const ap_array3* in_desc, const void* in,
         AP_REQUIRE(in_desc->shape[a] == out_desc->shape[a], "in and out shapes differ");
     AP_REQUIRE(in != out, "in and out must not overlap");
     AP_REQUIRE(physpixwid > 0.0, "physpixwid must be positive");
-    AP_REQUIRE(kernel == AP_RAMP_RAMLAK || kernel == AP_RAMP_SINC, "unknown ramp kernel %d", kernel);
+    AP_REQUIRE(kernel == AP_RAMP_RAMLAK || kernel == AP_RAMP_SINC || kernel == AP_RAMP_HILBERT ||
+                   kernel == AP_RAMP_HILBERT_HALF,
+               "unknown ramp kernel %d", kernel);
     const uint32_t V = in_desc->shape[0], nu = in_desc->shape[1], nv = in_desc->shape[2];
     if (V == 0 || nu == 0 || nv == 0) return AP_OK;
     AP_REQUIRE(nu <= (uint32_t)kMaxNu, "nu above %d", kMaxNu);

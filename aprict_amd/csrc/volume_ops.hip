@@ -345,7 +345,8 @@ int32_t ap_sirt_iterate(const ap_config_cb_backproject* bp_cfg, const ap_proj_sp
     int32_t st = ap_project_voxel(&fp, specs, num_projs, chunk, vol_desc, x, sino_desc, r, b, stream);
     if (st) return st;
     // x += M (.) BP(r)
-    return ap_backproject(bp_cfg, specs, num_projs, chunk, vol_desc, x, m_desc, M, sino_desc, r, stream);
+    return ap_backproject(bp_cfg, specs, num_projs, chunk, vol_desc, x, m_desc, M, sino_desc, r, nullptr, nullptr,
+                          stream);
 }
 
 }  // extern "C"

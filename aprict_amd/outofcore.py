@@ -313,10 +313,14 @@ def backproject_streamed(config: cbfp.ConfigBackproject, volume: HostVolume,
                          views_per_slice: int | None = None, device=None) -> None:
     """Reference backprojection (``flatpanel.backproject``) into a host volume, brick by brick."""
     cfg = config.contents
+    if int(cfg["backproj_weighting_preset"]) == 7:
+        msg = "the GBC backprojection (preset 7) writes a secondary volume; stream it with flatpanel.backproject"
+        raise NotImplementedError(msg)
 
     def call(lib, ps, data, chunk, vd, base, md, mbase):
         _abi.check(lib.ap_backproject(_abi.struct_ptr(cfg), _abi.ptr(ps.specs_device()), data.shape[0],
                                       _abi.struct_ptr(chunk), vd, base, md, mbase, _abi.array3(data),
-                                      _abi.ptr(data), _abi.stream_handle(data.device)), "backproject_streamed")
+                                      _abi.ptr(data), None, None, _abi.stream_handle(data.device)),
+                   "backproject_streamed")
 
     _backproject_streamed(call, volume, projs, brick, views_per_slice, device, summand_multiplier)

@@ -43,6 +43,13 @@ extern "C" {
 /* ---- ramp filter kernels ------------------------------------------------ */
 #define AP_RAMP_RAMLAK   0   /* ramp_kernels.py:43-59 */
 #define AP_RAMP_SINC     1   /* ramp_kernels.py:17-40 (ramp_kernel) */
+#define AP_RAMP_HILBERT  2   /* ramp_kernels.py:62-77 (hilbert_transform_kernel) */
+#define AP_RAMP_HILBERT_HALF 3   /* ramp_kernels.py:80-94 (..._halfoffset) */
+
+/* detector weighting modes of the FBP filter (ap_fbp_rotate_weight) */
+#define AP_FBP_GBC   0   /* cone weight + letterbox (ramp_filter.py:437-449, 689-754) */
+#define AP_FBP_FAN   1   /* cone weight, no letterbox (_hacky_fan_beam_mode)          */
+#define AP_FBP_FDK   2   /* cosine weight, no letterbox (_hacky_fdk_mode, :425-436)   */
 
 /* ---- reference kernel-argument structs --------------------------------- */
 typedef struct { float x, y, z; } ap_fvec3;
@@ -182,13 +189,20 @@ int32_t ap_project_voxel(const ap_config_cb_project* cfg,
  * (kernel cuda/cusrc_templates/CB_backproject.cu:952-1027; exact branch
  * :408-477, fast branch :478-691, softening :694-713).  For every voxel i of
  * the chunk:  vol[i] += (multiplier ? multiplier[i] : 1) * BP_cfg(sino)[i].
- * Weighting presets 1..7 (GBC) are AP_ERR_UNSUPPORTED.                      */
+ * backproj_weighting_preset 1-4 and 6 weight the direct ray-casting branch
+ * (:528-600; R = Z_on_val_factor_val).  Preset 7 replaces the whole
+ * backprojection by the GBC one for a cylindrical source locus of radius
+ * R about the z axis (backproject_CB_voltex_GBC_cyl, :736-1108): vol gets
+ * sum_views w * detector value, `secondary` (required, same dtype and shape
+ * as vol) gets sum_views w, both times the multiplier.  `secondary` is
+ * ignored for the other presets, as in the reference.                      */
 int32_t ap_backproject(const ap_config_cb_backproject* cfg,
                        const ap_proj_spec* specs, uint32_t num_projs,
                        const ap_vol_tex_chunk_spec* chunk,
                        const ap_array3* vol_desc, void* vol,
                        const ap_array3* mult_desc, const void* multiplier,
                        const ap_array3* sino_desc, const void* sino,
+                       const ap_array3* sec_desc, void* secondary,
                        void* stream);
 
 /* ---- matched adjoint of ap_project_voxel --------------------------------
@@ -249,6 +263,52 @@ int32_t ap_ramp_filter_rows(const ap_array3* in_desc, const void* in,
                             const ap_array3* out_desc, void* out,
                             double physpixwid, int32_t kernel,
                             int32_t fdk_cosine_weight, void* stream);
+
+/* ---- FBP detector rotation (GBC sampling) -------------------------------
+ * The per-projection steps of FBPFilterer.filter_bulk_axis1_stream
+ * (cb/ramp_filter.py:577-886) around the row filter, for in [V][nu][nv]
+ * and the expanded buffer [V][Eu][Ev] (Eu - nu and Ev - nv even, the
+ * padding split equally on both sides):
+ *
+ * ap_fbp_rotate_weight: out[p] = W_p * rot(-zeta_p) pad(in[p])  (:612-754)
+ *   pad: zero padding to Eu x Ev; rot(a): rotation by angle a (radians)
+ *   about the buffer centre ((Eu-1)/2, (Ev-1)/2), i.e. out(o) = in(M(o - c)
+ *   + c) with M = [[cos a, sin a], [-sin a, cos a]], sampled bilinearly
+ *   with zeros outside (scipy.ndimage.rotate order=1, mode grid-constant,
+ *   reshape=False); zeta_p = angles[p] (device float[V]; NULL = all 0;
+ *   zeta 0 copies exactly).  W_p(u index i, v index k): with
+ *   u = pw (i - (Eu-1)/2), v = pw (k - (Ev-1)/2), pw = pix_over_L,
+ *     AP_FBP_GBC, AP_FBP_FAN: 1 / ((1 + v^2)^(3/2) sqrt(1 + u^2 + v^2))
+ *     AP_FBP_FDK:             1 / sqrt(1 + u^2 + v^2)
+ *   and for AP_FBP_GBC times the letterbox of zeta_p (:470-501, 689-754):
+ *   the v range [m - d, m + d], m = (Ev-1)/2,
+ *   d = sqrt(max(cos^2 zeta_p / cos^2 zeta_max - 1, 0)) / pw, columns
+ *   outside round(m - d) .. round(m + d) zeroed and the end ones weighted by their
+ *   covered fraction.
+ * ap_fbp_unrotate: out[p] = crop(rot(+zeta_p) in[p])  (:807-886), the
+ *   inverse rotation of the filtered expanded buffer, cropped back to
+ *   nu x nv.
+ * in and out are F32 and must not overlap.                                 */
+int32_t ap_fbp_rotate_weight(const ap_array3* in_desc, const void* in,
+                             const ap_array3* out_desc, void* out,
+                             const float* angles, double pix_over_L,
+                             double zeta_max, int32_t mode, void* stream);
+int32_t ap_fbp_unrotate(const ap_array3* in_desc, const void* in,
+                        const ap_array3* out_desc, void* out,
+                        const float* angles, void* stream);
+
+/* ---- GBC weight integral -------------------------------------------------
+ * Replaces get_GBC_weight_integ (cuda/cusrc_templates/CB_backproject.cu:
+ * 1111-1168, host gbc/weightinteg.py:24-59): for each r = r_on_R[i]
+ * (device float[n], increasing), adds to out[i] (device double[n])
+ *   4 pi zmax / (nz nt) * sum_{a<nt} sum_{b<nz} h_b w(a, b)
+ * with w the preset-7 weight (R = 1, soft_vartheta_* of cfg) of the point
+ * (r, 0, 0) seen from the source (cos t_a, sin t_a, z_b), t_a = pi a / nt,
+ * z_b = zmax b / (nz - 1), zmax = tan(soft_vartheta_v) (1 + r_on_R[n-1]),
+ * h_0 = 1/2 and h_b = 1 otherwise.  nz >= 2.                              */
+int32_t ap_gbc_weight_integ(const float* r_on_R, uint32_t n, double* out,
+                            uint32_t num_z_samps, uint32_t num_ang_samps,
+                            const ap_config_cb_backproject* cfg, void* stream);
 
 /* ---- element-wise volume / sinogram operations --------------------------
  * Replace the texture kernels of cuda/cusrc_templates/texture_operations.cu

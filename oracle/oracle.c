@@ -512,6 +512,62 @@ static real det_nearest(const real* img, uint32_t nu, uint32_t nv, real fj, real
                     (long)floor((double)(fk + (real)0.5)));
 }
 
+/* Weighting presets of the direct ray-casting branch (CB_backproject.cu:
+ * 528-600): the source moves on a cylinder of radius R about the z axis;
+ * ell = |horizontal part of vp - S| / R, c = clamp(|Z / R / ell|, 0, 1),
+ * d = c - ell, vy and vx the components of vp across and along the source
+ * direction (units R^2).  Presets other than 1-4 and 6 leave mult alone. */
+static real preset_mult(int preset, real R, v3 S, v3 vp, real Z, real mult) {
+    v3 r = sub(vp, S);
+    real ell = (real)sqrt((double)(r.x * r.x + r.y * r.y)) / R;
+    real zr = Z / R;
+    real c = rmax(0, rmin(1, (real)fabs((double)(zr / ell))));
+    real d = c - ell;
+    real vy = (S.x * vp.y - S.y * vp.x) / R / R;
+    real vx = (S.x * vp.x + S.y * vp.y) / R / R;
+    real q = d / c, f = (1 - q * q) / (1 + q * q);
+    switch (preset) {
+    case 1: return mult / (c * c * (c * c + d * d));
+    case 2: return mult * (-2 * f * f * (-vy) * (1 + vx / zr) / (ell * ell));
+    case 3: return mult / (zr * c * c);
+    case 4: return mult * ((vy / zr) / zr);
+    case 6: return mult * (-2 * f * f / (1 - q) * (-vy) * (1 + vx / zr) / zr);
+    default: return mult;
+    }
+}
+
+/* s(a, b; c) of the GBC paper's appendix (CB_backproject.cu:274-311) */
+static real cossq_soft(real a, real b, real c) {
+    real ac = (real)fabs((double)c);
+    if (ac > a) return 0;
+    if (ac < a - b) return 1;
+    real g = 1 / ((real)sin((double)(a - b)) - (real)sin((double)a));
+    real x = g * (real)sin((double)ac) - g * (real)sin((double)a);
+    return rmax(0, rmin(1, 3 * x * x - 2 * x * x * x));
+}
+
+/* GBC weight of the source S for the point x, cylinder of radius R about the
+ * z axis (CB_backproject.cu:736-790, eq. (23) of the GBC paper) */
+static real gbc_weight(real R, real va, real vb, real ha, real hb, v3 S, v3 x) {
+    v3 r = sub(x, S);
+    real ln = (real)sqrt((double)(S.x * S.x + S.y * S.y));
+    real Z = (r.x * -S.x + r.y * -S.y) / ln;
+    real ell = (real)sqrt((double)(r.x * r.x + r.y * r.y)) / R;
+    real c = rmax(0, rmin(1, (real)fabs((double)(Z / R / ell))));
+    real d = c - ell;
+    real th = (real)atan2((double)(r.z / R), (double)ell);
+    real ce = (real)cos((double)th);
+    real den = c * c + d * d;
+    real w = ce * ce * ce * c / (den > (real)1e-9 ? den : (real)1e-9);
+    return w * cossq_soft(va, vb, th) * cossq_soft(ha, hb, (real)acos((double)c));
+}
+
+real OR_SUFFIX(or_gbc_weight)(const ap_config_cb_backproject* cfg, const double* S3, const double* x3) {
+    return gbc_weight((real)cfg->Z_on_val_factor_val, cfg->soft_vartheta_v, cfg->soft_vartheta_v_softspan,
+                      cfg->soft_vartheta_h, cfg->soft_vartheta_h_softspan, mk((real)S3[0], (real)S3[1], (real)S3[2]),
+                      mk((real)x3[0], (real)x3[1], (real)x3[2]));
+}
+
 /* contribution of one view to one (sub-)voxel at vox_pos */
 static real bp_view(const ap_config_cb_backproject* cfg, const ap_proj_spec* ps,
                     const real* img, v3 vp, real w) {
@@ -557,13 +613,17 @@ static real bp_view(const ap_config_cb_backproject* cfg, const ap_proj_spec* ps,
         real mult;
         if (cfg->approx_mode_direct_raycast) {
             mult = 1;
-            if (cfg->Z_on_L_factor_power != 0 || cfg->Z_on_val_factor_power != 0) {
+            if (cfg->Z_on_L_factor_power != 0 || cfg->Z_on_val_factor_power != 0 ||
+                cfg->backproj_weighting_preset != 0) {
                 v3 L = sub(D, S);
                 real Z = dot(sub(vp, S), L) / nrm(L);
                 if (cfg->Z_on_L_factor_power != 0)
                     mult *= powint(Z / nrm(L), cfg->Z_on_L_factor_power);
                 if (cfg->Z_on_val_factor_power != 0)
                     mult *= powint(Z / (real)cfg->Z_on_val_factor_val, cfg->Z_on_val_factor_power);
+                if (cfg->backproj_weighting_preset != 0)
+                    mult = preset_mult(cfg->backproj_weighting_preset, (real)cfg->Z_on_val_factor_val, S, vp, Z,
+                                       mult);
             }
         } else {
             v3 P = add(D, add(scl(vf(ds->u), cu), scl(vf(ds->v), cv)));
@@ -605,7 +665,7 @@ int OR_SUFFIX(or_backproject)(const ap_config_cb_backproject* cfg,
                               const uint32_t* dims, real* vol, const real* mult,
                               const real* sino, int nthreads) {
     if (!cfg || !specs || !chunk || !dims || !vol || !sino) return AP_ERR_INVALID;
-    if (cfg->backproj_weighting_preset != 0) return AP_ERR_UNSUPPORTED;
+    if (cfg->backproj_weighting_preset == 7) return AP_ERR_UNSUPPORTED;   /* or_backproject_gbc */
     if (cfg->ssx == 0 || cfg->ssy == 0 || cfg->ssz == 0) return AP_ERR_INVALID;
     set_threads(nthreads);
     const real w = chunk->voxwid;
@@ -649,14 +709,106 @@ int OR_SUFFIX(or_backproject)(const ap_config_cb_backproject* cfg,
     return AP_OK;
 }
 
+/* GBC backprojection (backproject_CB_voltex_GBC_cyl, CB_backproject.cu:
+ * 795-890, 1033-1108): vol += mult * mean_ss sum_views w * detector value,
+ * sec += mult * mean_ss sum_views w, with w = gbc_weight and the detector
+ * read bilinearly or at the nearest pixel. */
+int OR_SUFFIX(or_backproject_gbc)(const ap_config_cb_backproject* cfg,
+                                  const ap_proj_spec* specs, uint32_t nprojs,
+                                  const ap_vol_tex_chunk_spec* chunk,
+                                  const uint32_t* dims, real* vol, real* sec, const real* mult,
+                                  const real* sino, int nthreads) {
+    if (!cfg || !specs || !chunk || !dims || !vol || !sec || !sino) return AP_ERR_INVALID;
+    if (cfg->ssx == 0 || cfg->ssy == 0 || cfg->ssz == 0) return AP_ERR_INVALID;
+    set_threads(nthreads);
+    const real w = chunk->voxwid, R = cfg->Z_on_val_factor_val;
+    const v3 tp = vf(chunk->texpos);
+    const uint32_t nu = specs[0].dspec.nu, nv = specs[0].dspec.nv;
+    const uint32_t cx = chunk->shape.x, cy = chunk->shape.y, cz = chunk->shape.z;
+    long long total = (long long)cx * cy;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long long ij = 0; ij < total; ij++) {
+        uint32_t i = (uint32_t)(ij / cy), j = (uint32_t)(ij % cy);
+        for (uint32_t k = 0; k < cz; k++) {
+            uint32_t gi = i + chunk->offsets.x, gj = j + chunk->offsets.y, gk = k + chunk->offsets.z;
+            v3 c = add(tp, scl(mk((real)gi + (real)0.5, (real)gj + (real)0.5, (real)gk + (real)0.5), w));
+            real vx = 0, wx = 0;
+            for (uint32_t si = 0; si < cfg->ssx; si++) {
+                real vy = 0, wy = 0;
+                for (uint32_t sj = 0; sj < cfg->ssy; sj++) {
+                    real vz = 0, wz = 0;
+                    for (uint32_t sk = 0; sk < cfg->ssz; sk++) {
+                        v3 vp = add(c, scl(mk(-(real)0.5 + ((real)0.5 + si) / cfg->ssx,
+                                              -(real)0.5 + ((real)0.5 + sj) / cfg->ssy,
+                                              -(real)0.5 + ((real)0.5 + sk) / cfg->ssz), w));
+                        for (uint32_t p = 0; p < nprojs; p++) {
+                            const ap_proj_spec* ps = &specs[p];
+                            real g = gbc_weight(R, cfg->soft_vartheta_v, cfg->soft_vartheta_v_softspan,
+                                                cfg->soft_vartheta_h, cfg->soft_vartheta_h_softspan,
+                                                vf(ps->sourcepos), vp);
+                            wz += g;
+                            if (g == 0) continue;
+                            real cu, cv;
+                            det_coords(ps, vp, &cu, &cv);
+                            const real* img = sino + (size_t)p * nu * nv;
+                            real fj = cu + (real)(nu - 1) / 2, fk = cv + (real)(nv - 1) / 2;
+                            real y = cfg->approx_mode_linear_interpolate ? det_bilinear(img, nu, nv, fj, fk)
+                                                                         : det_nearest(img, nu, nv, fj, fk);
+                            vz += y * g;
+                        }
+                    }
+                    vy += vz / cfg->ssz;
+                    wy += wz / cfg->ssz;
+                }
+                vx += vy / cfg->ssy;
+                wx += wy / cfg->ssy;
+            }
+            size_t off = ((size_t)gi * dims[1] + gj) * dims[2] + gk;
+            real m = (cfg->apply_summand_multiplier && mult) ? mult[off] : 1;
+            vol[off] += m * (vx / cfg->ssx);
+            sec[off] += m * (wx / cfg->ssx);
+        }
+    }
+    return AP_OK;
+}
+
+/* Integral of the GBC weight over the source cylinder (R = 1) for points
+ * (r_i, 0, 0) (get_GBC_weight_integ, CB_backproject.cu:1111-1168): out[i] +=
+ * 4 pi zmax / (nz nt) sum_{a<nt, b<nz} h_b w(t_a, z_b), t_a = pi a / nt,
+ * z_b = zmax b / (nz - 1), zmax = tan(vartheta_v) (1 + r_{n-1}), h_0 = 1/2. */
+int OR_SUFFIX(or_gbc_weight_integ)(const double* r, uint32_t n, double* out, uint32_t nz, uint32_t nt,
+                                   const ap_config_cb_backproject* cfg) {
+    if (!r || !out || !cfg || nz < 2 || nt < 1) return AP_ERR_INVALID;
+    if (n == 0) return AP_OK;
+    const double pi = 3.14159265358979323846;
+    const double zmax = tan((double)cfg->soft_vartheta_v) * (1 + r[n - 1]);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < (long)n; i++) {
+        double acc = 0;
+        for (uint32_t a = 0; a < nt; a++) {
+            double t = pi * a / nt;
+            for (uint32_t b = 0; b < nz; b++) {
+                v3 S = mk((real)cos(t), (real)sin(t), (real)(zmax * b / (nz - 1)));
+                real g = gbc_weight(1, cfg->soft_vartheta_v, cfg->soft_vartheta_v_softspan, cfg->soft_vartheta_h,
+                                    cfg->soft_vartheta_h_softspan, S, mk((real)r[i], 0, 0));
+                acc += (b == 0 ? 0.5 : 1.0) * (double)g;
+            }
+        }
+        out[i] += 4 * pi * zmax / ((double)nz * nt) * acc;
+    }
+    return AP_OK;
+}
+
 /* ======================================================================
  * Ramp filter: direct linear convolution along axis 1 (u) with the taps
- * of ramp_kernels.py:17-59, optional FDK cosine pre-weight
+ * of ramp_kernels.py:27-94, optional FDK cosine pre-weight
  * (cb/ramp_filter.py:407-436).  Accumulated in long double.
  * ====================================================================== */
 static long double ramp_tap(long n, long double p, int kernel) {
     const long double pi = 3.141592653589793238462643383279502884L;
     if (kernel == AP_RAMP_SINC) return 8.0L / (1.0L - 4.0L * (long double)n * n) / p;
+    if (kernel == AP_RAMP_HILBERT) return n == 0 ? 0 : 2.0L / (long double)n;   /* :62-77 */
+    if (kernel == AP_RAMP_HILBERT_HALF) return 2.0L / ((long double)n + 0.5L);  /* :80-94 */
     if (n == 0) return pi * pi / p;
     if (n % 2 == 0) return 0;
     return -4.0L / ((long double)n * n) / p;

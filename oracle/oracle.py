@@ -39,8 +39,10 @@ def _lib(fp32: bool) -> ctypes.CDLL:
         real = ctypes.c_float if fp32 else ctypes.c_double
         sfx = "_f32" if fp32 else ""
         getattr(lib, "or_slab_length" + sfx).restype = real
+        getattr(lib, "or_gbc_weight" + sfx).restype = real
         for name in ("or_project_voxel", "or_backproject", "or_backproject_matched", "or_ramp_filter",
-                     "or_project_forbild", "or_voxelise_forbild", "or_project_voxel_refdda"):
+                     "or_project_forbild", "or_voxelise_forbild", "or_project_voxel_refdda", "or_backproject_gbc",
+                     "or_gbc_weight_integ"):
             getattr(lib, name + sfx).restype = ctypes.c_int
         _LIBS[fp32] = lib
     return _LIBS[fp32]
@@ -105,6 +107,36 @@ def backproject(cfg, specs, chunk, vol, sino, mult=None, *, nthreads: int = 0, f
                                      _p(_dims(vol)), _p(vol), _p(m), _p(sino), ctypes.c_int(nthreads))
     _ok(st, "backproject")
     return vol
+
+
+def backproject_gbc(cfg, specs, chunk, vol, sec, sino, mult=None, *, nthreads: int = 0, fp32: bool = False):
+    """(vol + mult * sum_views w y, sec + mult * sum_views w): the preset-7 GBC backprojection."""
+    vol = _dense(vol, fp32).copy()
+    sec = _dense(sec, fp32).copy()
+    sino = _dense(sino, fp32)
+    m = _dense(mult, fp32) if mult is not None else None
+    specs = np.ascontiguousarray(specs).reshape(-1)
+    st = _fn("or_backproject_gbc", fp32)(_p(cfg), _p(specs), ctypes.c_uint32(len(specs)), _p(chunk),
+                                         _p(_dims(vol)), _p(vol), _p(sec), _p(m), _p(sino), ctypes.c_int(nthreads))
+    _ok(st, "backproject_gbc")
+    return vol, sec
+
+
+def gbc_weight(cfg, S, x) -> float:
+    """Preset-7 weight of the source S for the point x (R = Z_on_val_factor_val)."""
+    s3 = np.ascontiguousarray(S, dtype=np.float64)
+    x3 = np.ascontiguousarray(x, dtype=np.float64)
+    return float(_fn("or_gbc_weight", False)(_p(cfg), _p(s3), _p(x3)))
+
+
+def gbc_weight_integ(r_on_R, cfg, num_z_samps: int, num_ang_samps: int) -> np.ndarray:
+    """Quadrature of the preset-7 weight over the source cylinder (R = 1) for points (r, 0, 0)."""
+    r = np.ascontiguousarray(r_on_R, dtype=np.float64)
+    out = np.zeros_like(r)
+    st = _fn("or_gbc_weight_integ", False)(_p(r), ctypes.c_uint32(len(r)), _p(out), ctypes.c_uint32(num_z_samps),
+                                           ctypes.c_uint32(num_ang_samps), _p(cfg))
+    _ok(st, "gbc_weight_integ")
+    return out
 
 
 def backproject_matched(cfg, specs, chunk, vol, sino, mult=None, alpha: float = 1.0, *, fp32: bool = False):

@@ -54,6 +54,7 @@ def _desc(shape, dtype=_abi.AP_F32):
 
 
 FAKE = ctypes.c_void_p(0x1000)   # never dereferenced: every call below fails validation first
+FAKE2 = ctypes.c_void_p(0x2000)
 
 
 def test_invalid_arguments_are_reported_not_launched():
@@ -74,13 +75,25 @@ def test_invalid_arguments_are_reported_not_launched():
     st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE,
                               _desc((3, 5, 5), _abi.AP_BF16), FAKE, None, None)
     assert st == _abi.AP_ERR_UNSUPPORTED
-    # GBC presets are refused
-    bp = cbfp.ConfigBackproject(backproj_weighting_preset=7).contents
-    st = lib.ap_backproject(_abi.struct_ptr(bp), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, None, None, sd, FAKE, None)
-    assert st == _abi.AP_ERR_UNSUPPORTED and b"GBC" in lib.ap_last_error()
+    # the GBC preset needs a secondary volume of the volume's shape and dtype
+    bp = cbfp.ConfigBackproject(backproj_weighting_preset=7, Z_on_val_factor_val=500.0).contents
+    st = lib.ap_backproject(_abi.struct_ptr(bp), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, None, None, sd, FAKE,
+                            None, None, None)
+    assert st == _abi.AP_ERR_INVALID and b"secondary" in lib.ap_last_error()
+    st = lib.ap_backproject(_abi.struct_ptr(bp), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, None, None, sd, FAKE,
+                            _desc((4, 4, 3)), FAKE2, None)
+    assert st == _abi.AP_ERR_INVALID and b"shape" in lib.ap_last_error()
     # multiplier flag without a multiplier
     bp = cbfp.ConfigBackproject(apply_summand_multiplier=True).contents
-    st = lib.ap_backproject(_abi.struct_ptr(bp), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, None, None, sd, FAKE, None)
+    st = lib.ap_backproject(_abi.struct_ptr(bp), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, None, None, sd, FAKE,
+                            None, None, None)
+    assert st == _abi.AP_ERR_INVALID
+    # FBP rotation: odd padding, unknown mode; weight integral: too few heights
+    st = lib.ap_fbp_rotate_weight(sd, FAKE, _desc((3, 6, 6)), FAKE2, None, 0.01, 0.1, 0, None)
+    assert st == _abi.AP_ERR_INVALID and b"odd padding" in lib.ap_last_error()
+    st = lib.ap_fbp_rotate_weight(sd, FAKE, _desc((3, 7, 7)), FAKE2, None, 0.01, 0.1, 9, None)
+    assert st == _abi.AP_ERR_INVALID and b"mode" in lib.ap_last_error()
+    st = lib.ap_gbc_weight_integ(FAKE, 4, FAKE2, 1, 10, _abi.struct_ptr(bp), None)
     assert st == _abi.AP_ERR_INVALID
     # ramp filter: in == out
     st = lib.ap_ramp_filter_rows(sd, FAKE, sd, FAKE, 1.0, 0, 0, None)
