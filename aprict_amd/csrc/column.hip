@@ -335,6 +335,7 @@ constexpr int kWCols = 8;               // sub-columns per wave pass
 // segments of one column inside a brick (<= kTX + kTY - 1) plus 3 padding
 // entries, rounded to whole float4 rows
 constexpr int kListCap = kTX + kTY + 4;
+constexpr int kTXY = kTX > kTY ? kTX : kTY;   // most crossings of one axis in a brick
 
 // What the rows of one sub-column need, and the state that builds its list.
 struct ColEntry {
@@ -576,8 +577,8 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
             }
             wave_sync_lds();
             // merged x-y crossing lists: (column, crossing) pairs across the wave
-            for (int e = lane; e < nc * kTX; e += 64) {
-                const int g = e / kTX, m = e % kTX;
+            for (int e = lane; e < nc * kTXY; e += 64) {
+                const int g = e / kTXY, m = e % kTXY;
                 const ColEntry& E = W.ce[g];
                 if (m < E.qx.n) {
                     const float h = seq_at(E.qx, m);
