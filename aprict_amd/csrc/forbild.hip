@@ -34,7 +34,35 @@ struct Prim {
     int clip_mode[3];
     float rho;
     int cls;
+    float bc[3], br2;   // bounding sphere (centre, squared radius) for culling; br2 NaN: never culled
 };
+
+// Radius of a sphere about c holding the primitive: |M^-1 l| <= sigma_max(M^-1) |l|
+// for the local point l, |l| <= lr (the local shape's radius), and sigma_max^2 =
+// lambda_max((M^-1)^T M^-1) <= its largest absolute row sum (Gershgorin; exact
+// when the frame is orthonormal, since the matrix is then diagonal).
+__device__ void set_bound(Prim& P, double lr) {
+    const double* m = P.M;
+    const double c00 = m[4] * m[8] - m[5] * m[7], c01 = m[5] * m[6] - m[3] * m[8], c02 = m[3] * m[7] - m[4] * m[6];
+    const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+    const double id = 1.0 / det;
+    // A = M^-1 = adj(M) / det
+    const double A[9] = {c00 * id, (m[2] * m[7] - m[1] * m[8]) * id, (m[1] * m[5] - m[2] * m[4]) * id,
+                         c01 * id, (m[0] * m[8] - m[2] * m[6]) * id, (m[2] * m[3] - m[0] * m[5]) * id,
+                         c02 * id, (m[1] * m[6] - m[0] * m[7]) * id, (m[0] * m[4] - m[1] * m[3]) * id};
+    double lam = 0.0;
+    for (int i = 0; i < 3; i++) {
+        double row = 0.0;
+        for (int j = 0; j < 3; j++)
+            row += fabs(A[i] * A[j] + A[3 + i] * A[3 + j] + A[6 + i] * A[6 + j]);
+        lam = fmax(lam, row);
+    }
+    const double r = sqrt(lam) * lr * (1.0 + 1e-6);
+    P.bc[0] = (float)P.c[0];
+    P.bc[1] = (float)P.c[1];
+    P.bc[2] = (float)P.c[2];
+    P.br2 = (float)(r * r);   // inf or NaN (degenerate shape) never culls
+}
 
 __device__ __forceinline__ d3 unit_of(ap_fvec3 a) {
     d3 v = fv(a);
@@ -105,6 +133,12 @@ __device__ void make_prim(const ap_forbild_shape& s, Prim& P) {
         P.clip_mode[i] = cs[i]->mode;
     }
     P.rho = s.rho;
+    // local radius: unit cube corner, unit ball, unit disk x [-h, h], cone of radius cr x [-h, h]
+    const double lr = P.cls == kCube ? sqrt(3.0)
+                    : P.cls == kBall ? 1.0
+                    : P.cls == kCyl  ? sqrt(1.0 + P.h * P.h)
+                    : P.cls == kCone ? sqrt(P.cr * P.cr + P.h * P.h) : 0.0;
+    set_bound(P, lr);
 }
 
 // stable roots of A t^2 + 2 B t + C = 0; false if no real interval
@@ -265,50 +299,103 @@ __device__ bool prim_contains(const Prim& P, const d3& p) {
     }
 }
 
-constexpr int kProjThreads = 64;   // one wave per block
+constexpr int kProjThreads = 256;  // 4 waves share the staged shapes
 constexpr int kShapeBatch = 64;    // shapes staged in LDS at a time
-constexpr int kSlots = 24;         // stored intervals per ray before the exact fallback
+constexpr int kUnion = 8;          // disjoint covered pieces kept per ray before the exact fallback
 constexpr uint32_t kRefMaxIntersections = 1000;   // reference's overflow threshold
 
-struct Slot { double a, b; float rho; int k; };
+// Last-wins compositing (FORBILD_Shape.cu:474-540): the density at a point is
+// that of the highest-index shape containing it, so the integral is
+//   sum_q rho_q |I_q \ (union of I_q' for q' > q)|.
+// Shapes are visited from the last to the first; the union of the intervals
+// seen so far is kept as at most kUnion disjoint pieces in registers (empty
+// pieces are [3, 2]).  Adding an interval I: its covered length is the sum of
+// its overlaps with the pieces; then every piece it overlaps or touches is
+// absorbed into it (one pass suffices: the pieces are disjoint, so a piece
+// that meets the grown interval meets I itself) and the result takes a free
+// piece.  Returns false when no piece is free.
+struct Union {
+    double a[kUnion], b[kUnion];
+};
 
-// last-wins integral over stored intervals (the highest shape index k that
-// covers a point decides its density)
-__device__ double sweep_last_wins(const Slot* s, int m) {
-    double acc = 0.0;
-    double prog = -1.0;
-    for (int guard = 0; guard < 4 * m + 4; guard++) {
-        // top interval covering prog (or the next start when none covers)
+__device__ __forceinline__ void union_clear(Union& U) {
+#pragma unroll
+    for (int i = 0; i < kUnion; i++) {
+        U.a[i] = 3.0;
+        U.b[i] = 2.0;
+    }
+}
+
+__device__ __forceinline__ bool union_add(Union& U, double ta, double tb, double& uncovered) {
+    double cov = 0.0, na = ta, nb = tb;
+#pragma unroll
+    for (int i = 0; i < kUnion; i++) {
+        cov += fmax(0.0, fmin(tb, U.b[i]) - fmax(ta, U.a[i]));
+        if (U.a[i] <= nb && U.b[i] >= na) {
+            na = fmin(na, U.a[i]);
+            nb = fmax(nb, U.b[i]);
+            U.a[i] = 3.0;
+            U.b[i] = 2.0;
+        }
+    }
+    uncovered = (tb - ta) - cov;
+    bool placed = false;
+#pragma unroll
+    for (int i = 0; i < kUnion; i++) {
+        if (!placed && U.a[i] > U.b[i]) {
+            U.a[i] = na;
+            U.b[i] = nb;
+            placed = true;
+        }
+    }
+    return placed;
+}
+
+// Exact last-wins integral recomputed from global memory at every event; for
+// rays whose covered set has more than kUnion pieces
+__device__ double last_wins_exact(const ap_forbild_shape* __restrict__ shapes, uint32_t K, const d3& S,
+                                  const d3& d) {
+    double val = 0.0, prog = -1.0;
+    for (uint32_t guard = 0; guard < 4 * K + 4; guard++) {
         int top = -1;
-        double nxt = 2.0;
-        for (int i = 0; i < m; i++) {
-            if (s[i].a <= prog && prog < s[i].b) {
-                if (top < 0 || s[i].k > s[top].k) top = i;
+        double topb = 0.0, nxt = 2.0;
+        float toprho = 0.f;
+        for (int q = (int)K - 1; q >= 0; q--) {
+            Prim P;
+            make_prim(shapes[q], P);
+            double ta, tb;
+            if (!prim_interval(P, S, d, ta, tb)) continue;
+            if (top < 0 && ta <= prog && prog < tb) {
+                top = q; topb = tb; toprho = P.rho;
+            } else if (top < 0 && ta > prog && ta < nxt) {
+                nxt = ta;   // candidate next start (no cover yet)
             }
         }
         if (top >= 0) {
-            nxt = s[top].b;
-            for (int i = 0; i < m; i++)
-                if (s[i].k > s[top].k && s[i].a > prog && s[i].a < nxt) nxt = s[i].a;
-            acc += (nxt - prog) * (double)s[top].rho;
-            prog = nxt;
+            double e = topb;
+            for (int q = (int)K - 1; q > top; q--) {
+                Prim P;
+                make_prim(shapes[q], P);
+                double ta, tb;
+                if (prim_interval(P, S, d, ta, tb) && ta > prog && ta < e) e = ta;
+            }
+            val += (e - prog) * (double)toprho;
+            prog = e;
         } else {
-            for (int i = 0; i < m; i++)
-                if (s[i].a > prog && s[i].a < nxt) nxt = s[i].a;
             if (nxt >= 2.0) break;
             prog = nxt;
         }
     }
-    return acc;
+    return val;
 }
 
+// 3 waves per SIMD: at most 168 VGPRs
 template <typename Tp>
-__global__ __launch_bounds__(kProjThreads) void k_project_forbild(
+__global__ __launch_bounds__(kProjThreads, 3) void k_project_forbild(
     float mba, int additive, uint32_t ssu, uint32_t ssv, const ap_proj_spec* __restrict__ specs,
     const ap_forbild_shape* __restrict__ shapes, uint32_t K, uint32_t nu, uint32_t nv,
     int64_t s0, int64_t s1, int64_t s2, Tp* __restrict__ sino, uint32_t* __restrict__ oom) {
     __shared__ Prim prims[kShapeBatch];
-    __shared__ Slot slots[kProjThreads][kSlots];
     const uint32_t pix = blockIdx.x * kProjThreads + threadIdx.x;
     const uint32_t p = blockIdx.y;
     const bool in = pix < nu * nv;
@@ -325,72 +412,49 @@ __global__ __launch_bounds__(kProjThreads) void k_project_forbild(
             const d3 Q = g.D + g.U * a + g.V * b;
             const d3 d = Q - g.S;
             const double len = sqrt(dot(d, d));
-            int m = 0;
+            const bool live = in && len > 0.0;
+            // the ray in FP32 for the bounding-sphere cull
+            const float Sf[3] = {(float)g.S.x, (float)g.S.y, (float)g.S.z};
+            const float df[3] = {(float)d.x, (float)d.y, (float)d.z};
+            const float idd = 1.f / (df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
             uint32_t nhit = 0;
             bool spill = false;
             double val = 0.0;
-            for (uint32_t k0 = 0; k0 < K; k0 += kShapeBatch) {
-                const uint32_t nb = min((uint32_t)kShapeBatch, K - k0);
+            Union U;
+            union_clear(U);
+            // batches from the last shape down, each visited last to first
+            for (uint32_t end = K; end > 0;) {
+                const uint32_t nb = min((uint32_t)kShapeBatch, end);
+                const uint32_t k0 = end - nb;
                 __syncthreads();
                 if (threadIdx.x < nb) make_prim(shapes[k0 + threadIdx.x], prims[threadIdx.x]);
                 __syncthreads();
-                if (!in || len == 0.0) continue;
-                for (uint32_t q = 0; q < nb; q++) {
+                end = k0;
+                if (!live || spill) continue;
+                for (int q = (int)nb - 1; q >= 0; q--) {
+                    // the line misses the bounding sphere: squared distance of
+                    // its centre from the line, with a margin of 1e-6 |v|^2 for
+                    // the FP32 cancellation (a wave skips the shape when all
+                    // its rays miss)
+                    const Prim& P = prims[q];
+                    const float vx = P.bc[0] - Sf[0], vy = P.bc[1] - Sf[1], vz = P.bc[2] - Sf[2];
+                    const float vd = vx * df[0] + vy * df[1] + vz * df[2];
+                    const float vv = vx * vx + vy * vy + vz * vz;
+                    if (vv - vd * vd * idd > P.br2 + 1e-6f * vv) continue;
                     double ta, tb;
-                    if (!prim_interval(prims[q], g.S, d, ta, tb)) continue;
-                    nhit++;
-                    if (nhit > kRefMaxIntersections) { overflow = true; continue; }
-                    if (additive) { val += (tb - ta) * (double)prims[q].rho; continue; }
-                    if (m < kSlots) {
-                        slots[threadIdx.x][m] = Slot{ta, tb, prims[q].rho, (int)(k0 + q)};
-                        m++;
-                    } else {
-                        spill = true;
-                    }
+                    if (!prim_interval(P, g.S, d, ta, tb)) continue;
+                    // more than the reference's limit of intersections: flag
+                    // it and ignore the rest (the reference drops the
+                    // highest-index ones, this loop the lowest)
+                    if (++nhit > kRefMaxIntersections) { overflow = true; continue; }
+                    const double rho = (double)prims[q].rho;
+                    if (additive) { val += (tb - ta) * rho; continue; }
+                    double un;
+                    if (!union_add(U, ta, tb, un)) { spill = true; break; }
+                    val += un * rho;
                 }
             }
-            if (in && len > 0.0 && !additive) {
-                if (!spill) {
-                    val = sweep_last_wins(slots[threadIdx.x], m);
-                } else {
-                    // more intersecting shapes than slots: exact sweep that
-                    // re-derives intervals from global memory at every event
-                    double prog = -1.0;
-                    for (uint32_t guard = 0; guard < 4 * nhit + 4; guard++) {
-                        int top = -1;
-                        double topb = 0.0, nxt = 2.0;
-                        float toprho = 0.f;
-                        for (int q = (int)K - 1; q >= 0; q--) {
-                            Prim P;
-                            make_prim(shapes[q], P);
-                            double ta, tb;
-                            if (!prim_interval(P, g.S, d, ta, tb)) continue;
-                            if (top < 0 && ta <= prog && prog < tb) {
-                                top = q; topb = tb; toprho = P.rho;
-                            } else if (top < 0 && ta > prog && ta < nxt) {
-                                nxt = ta;   // candidate next start (no cover yet)
-                            } else if (top >= 0 && ta > prog) {
-                                // shapes below the top cannot interrupt it
-                            }
-                            (void)topb;
-                        }
-                        if (top >= 0) {
-                            double e = topb;
-                            for (int q = (int)K - 1; q > top; q--) {
-                                Prim P;
-                                make_prim(shapes[q], P);
-                                double ta, tb;
-                                if (prim_interval(P, g.S, d, ta, tb) && ta > prog && ta < e) e = ta;
-                            }
-                            val += (e - prog) * (double)toprho;
-                            prog = e;
-                        } else {
-                            if (nxt >= 2.0) break;
-                            prog = nxt;
-                        }
-                    }
-                }
-            }
+            if (live && spill) val = last_wins_exact(shapes, K, g.S, d);
             accu += val * len;
         }
         tot += accu / ssv;

@@ -14,6 +14,7 @@ the reference semantics timed on a bounded sample of the same workload.
     python bench.py [--steps K] [--warmup W] [--volume-dtype f32|bf16]
     python bench.py --calibrate-copy     # plain 1 GiB device copies, for
                                          # scaling rocprofv3 FETCH/WRITE_SIZE
+    python bench.py --analytic-only      # only the analytic FORBILD leg
 
 roofline.traffic (HBM bytes per launch of the dominant kernel) is read from
 profiles/hbm_traffic_siddon_512cube_720views.txt, the summary of two
@@ -53,6 +54,62 @@ def make_volume(n, w, n_ellipsoids, seed, dtype):
         vb.data.copy_(v.data)
         return vb
     return v
+
+
+FP64_PEAK_TFLOPS = 78.6
+FP64_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "fp64_mix_forbild_cfg3.txt")
+
+
+def read_fp64_flops(kernel: str):
+    """FP64 flops per launch of ``kernel`` from the committed SQ-counter summary, or None."""
+    try:
+        with open(FP64_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "fp64_flops_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        return None
+    return None
+
+
+def analytic_leg(reps: int = 3) -> dict:
+    """Analytic FORBILD projection (ap_project_forbild) of 40 random clipped
+    primitives, 720 views x 512^2 at 0.5 mm: ray-primitive tests per second and,
+    with the FP64 instruction counts of the committed profile, the fraction of the
+    FP64 vector peak."""
+    import torch
+
+    from aprict_amd import phantoms as ph
+    from aprict_amd.cb import flatpanel as cbfp
+    from aprict_amd.volume import FBVolume
+
+    c = ph.CONFIGS[3]
+    V, nu, nv, k = c["views"], c["nu"], c["nv"], c["n_primitives"]  # noqa: N806
+    fb = FBVolume(ph.random_primitives(k, c["scale"], c["seed"]))
+    pg = ph.circular_geometry(V, nu, nv, c["pitch"])
+    ps = cbfp.ProjectionSet(pg, data=torch.zeros((V, nu, nv), dtype=torch.float32, device="cuda"))
+    cfg = cbfp.ConfigProject()
+    stream = torch.cuda.current_stream()
+    cbfp.project(cfg, fb, ps)   # warm-up (uploads the shapes and specs)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        ps.data.zero_()
+        ev = time_events(lambda: cbfp.project(cfg, fb, ps), stream)
+        torch.cuda.synchronize()
+        times.append(ev[0].elapsed_time(ev[1]) / 1e3)
+    t = min(times)
+    rays = V * nu * nv
+    out = {"workload": f"{k} FORBILD primitives (ellipsoids, elliptic cylinders, boxes, 0-3 clip planes), "
+                       f"{V} views, {nu}x{nv} detector at {c['pitch']} mm", "ms": t * 1e3,
+           "rays_per_s": rays / t, "ray_primitive_tests_per_s": rays * k / t}
+    flops = read_fp64_flops("k_project_forbild")
+    if flops is not None:
+        out["fp64"] = {"achieved": flops / t / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": flops / t / 1e12 / FP64_PEAK_TFLOPS,
+                       "model": "(2 FMA_F64 + ADD_F64 + MUL_F64 + TRANS_F64) x 64 per launch from rocprofv3 SQ counters"}
+    return out
 
 
 def time_events(fn, stream):
@@ -264,10 +321,16 @@ def main() -> int:
     ap.add_argument("--no-sirt", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-storage", action="store_true")
+    ap.add_argument("--no-analytic", action="store_true")
+    ap.add_argument("--analytic-only", action="store_true", help="run only the analytic FORBILD leg")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.analytic_only:
+        log("analytic FORBILD projection, 40 primitives, 720 x 512^2")
+        print(json.dumps({"analytic": analytic_leg()}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the projector shards by views across GPUs, which this build does not run)", file=sys.stderr)
@@ -371,6 +434,9 @@ def main() -> int:
     if not args.no_batched:
         log("batched SIRT: 64 problems, plain eager vs one graph")
         out["batched"] = batched_leg()
+    if not args.no_analytic:
+        log("analytic FORBILD projection, 40 primitives, 720 x 512^2")
+        out["analytic"] = analytic_leg()
     if not args.no_cpu_baseline:
         log(f"CPU baseline on {args.cpu_views} views")
         vnp = vol.data.float().cpu().numpy()

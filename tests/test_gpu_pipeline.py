@@ -109,9 +109,9 @@ def test_forbild_tapered_lung_tree_matches_oracle(device):
     assert err <= 1e-5
 
 
-def test_forbild_many_overlaps_use_exact_fallback(device):
-    # 60 nested spheres: every central ray meets more shapes than the
-    # per-ray interval slots, exercising the recomputing sweep
+def test_forbild_many_nested_overlaps(device):
+    # 60 nested spheres: central rays meet all of them; the covered set stays
+    # one piece while the shapes are composited from the last to the first
     shapes = forbild.stack([forbild.sphere((0.01 * i, 0, 0), 30.0 - 0.4 * i, 0.1 + 0.01 * i) for i in range(60)])
     pg = ph.circular_geometry(3, 16, 16, 4.0, R=500.0, L=1000.0)
     ps = cbfp.ProjectionSet(pg)
@@ -119,6 +119,23 @@ def test_forbild_many_overlaps_use_exact_fallback(device):
     want = orc.project_forbild(cbfp.ConfigProject(ssu=2).contents_fb, pg.to_proj_spec_array(), shapes,
                                np.zeros((3, 16, 16)))
     assert rel_l2(ps.data.cpu().numpy(), want) <= 1e-5
+
+
+def test_forbild_many_disjoint_pieces_use_exact_fallback(device):
+    # a large sphere (index 0) holding 14 small disjoint spheres on the x axis:
+    # rays along x meet the small ones first in compositing order, so their
+    # covered set grows to more pieces than the per-ray registers hold and the
+    # recomputing sweep takes over
+    small = [forbild.sphere((-19.5 + 3.0 * i, 0.0, 0.0), 1.2, 0.5 + 0.05 * i) for i in range(14)]
+    shapes = forbild.stack([forbild.sphere((0.0, 0.0, 0.0), 25.0, 0.2), *small])
+    pg = ph.circular_geometry(4, 24, 16, 0.5, R=500.0, L=1000.0)
+    for cfg in (cbfp.ConfigProject(), cbfp.ConfigProject(ssv=2)):
+        ps = cbfp.ProjectionSet(pg)
+        cbfp.project(cfg, FBVolume(shapes), ps, block=True)
+        want = orc.project_forbild(cfg.contents_fb, pg.to_proj_spec_array(), shapes, np.zeros((4, 24, 16)))
+        err = rel_l2(ps.data.cpu().numpy(), want)
+        print(f"disjoint pieces (ssv={cfg.contents['ssv']}) rel L2 {err:.3e}")
+        assert np.abs(want).max() > 10.0 and err <= 1e-5
 
 
 def test_voxeliser_and_bboxes(device):
