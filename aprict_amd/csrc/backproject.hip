@@ -79,6 +79,7 @@ struct BpCfg {
     float inv_sep;        // 1 / sep (0 when sep == 0)
     float w3;             // voxwid^3
     bool off32;           // detector offsets of one view fit 32 bits
+    bool pair;            // v contiguous (s2 == 1), nv >= 2, 32-bit offsets: rows k, k+1 in one load
     int zl, zv;
     uint32_t ssx, ssy, ssz;
     float w;              // voxwid
@@ -206,6 +207,12 @@ __device__ __forceinline__ float bp_fast_view(const BpCfg& c, const ViewMap& m, 
     return soften(c, m, cu, cv, sample_det(c, m, img, cu, cv) * mult);
 }
 
+// two detector values adjacent along v, read in one load (element alignment)
+template <typename Tp>
+struct alignas(sizeof(Tp)) TapPair {
+    Tp lo, hi;
+};
+
 // Detector value at (cu, cv) pixels from the centre: bilinear
 // (CB_backproject.cu:221-254) or nearest (:256-271), zero off the detector.
 template <typename Tp>
@@ -222,6 +229,25 @@ __device__ __forceinline__ float sample_det(const BpCfg& c, const ViewMap& m, co
             const float fl_j = floorf(fj), fl_k = floorf(fk);
             const int j0 = (int)fl_j, k0 = (int)fl_k;
             const float a = fj - fl_j, b = fk - fl_k;
+            if (c.pair) {
+                // rows k0, k0 + 1 of columns ja, jb from the pair starting at
+                // kc = clamp(k0, 0, nv - 2): kc = k0 inside the detector, and at
+                // its edges (k0 = -1 or nv - 1) the missing row reads 0
+                const int ja = max(j0, 0), jb = min(j0 + 1, (int)c.nu - 1);
+                const int kc = min(max(k0, 0), (int)c.nv - 2);
+                const TapPair<Tp> pa = *reinterpret_cast<const TapPair<Tp>*>(img + (ja * (int32_t)c.s1 + kc));
+                const TapPair<Tp> pb = *reinterpret_cast<const TapPair<Tp>*>(img + (jb * (int32_t)c.s1 + kc));
+                const bool klo = k0 < 0, khi = k0 + 1 >= (int)c.nv;
+                float y00 = klo ? 0.f : (khi ? to_f32(pa.hi) : to_f32(pa.lo));
+                float y01 = khi ? 0.f : (klo ? to_f32(pa.lo) : to_f32(pa.hi));
+                float y10 = klo ? 0.f : (khi ? to_f32(pb.hi) : to_f32(pb.lo));
+                float y11 = khi ? 0.f : (klo ? to_f32(pb.lo) : to_f32(pb.hi));
+                if (j0 < 0) { y00 = 0.f; y01 = 0.f; }
+                if (j0 + 1 >= (int)c.nu) { y10 = 0.f; y11 = 0.f; }
+                const float t1 = y00 * (1.f - a) + y10 * a;
+                const float t2 = y01 * (1.f - a) + y11 * a;
+                return t1 * (1.f - b) + t2 * b;
+            }
             // the four taps at clamped (in-range) indices, zeroed off the detector
             const int ja = max(j0, 0), jb = min(j0 + 1, (int)c.nu - 1);
             const int ka = max(k0, 0), kb = min(k0 + 1, (int)c.nv - 1);
@@ -577,6 +603,7 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
     c.w3 = c.w * c.w * c.w;
     c.off32 = (double)(c.nu ? c.nu - 1 : 0) * (double)c.s1 + (double)(c.nv ? c.nv - 1 : 0) * (double)c.s2 <
               2147483647.0;
+    c.pair = c.off32 && c.s2 == 1 && c.nv >= 2;
     const int var = ap_tune_variant(AP_TUNE_BACKPROJECT);
     const uint32_t rows_y = (uint32_t)kBpRowsY[var];
     dim3 block(kBz, rows_y, 1);

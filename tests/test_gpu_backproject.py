@@ -161,6 +161,39 @@ def test_reference_backprojector_matches_oracle(device, name):
     assert err <= (1e-5 if name.startswith("exact") else 2e-5)
 
 
+@pytest.mark.parametrize("layout", ["v_contiguous", "u_contiguous", "fp16"])
+def test_fast_backprojector_detector_edges(device, layout):
+    # a volume wider and taller than the field of view: many voxels project
+    # onto the detector's first and last rows and columns or just beyond
+    # them, where bilinear taps fall off the detector.  v-contiguous data
+    # reads the two rows of a tap column in one load; other layouts read the
+    # four taps one by one; fp16 data reads pairs of halves
+    import torch
+
+    rng = np.random.default_rng(31)
+    shape, w = (30, 30, 36), 0.6
+    pg = ph.circular_geometry(9, 20, 16, 1.0, R=40.0, L=80.0)
+    y = rng.uniform(-1, 1, (9, 20, 16)).astype(np.float32)
+    cfg = cbfp.ConfigBackproject(fast_approx=True, approx_mode_bilinear_interpolate=True)
+    if layout == "u_contiguous":
+        data = to_gpu(y).transpose(1, 2).contiguous().transpose(1, 2)   # strides (nu nv, 1, nu)
+        assert data.stride(2) != 1
+        yy = y
+    elif layout == "fp16":
+        data = to_gpu(y).to(torch.float16)
+        yy = data.float().cpu().numpy()
+    else:
+        data = to_gpu(y)
+        yy = y
+    v = Volume(np.zeros(shape, np.float32), voxwid=w)
+    cbfp.backproject(cfg, v, cbfp.ProjectionSet(pg, data=data), block=True)
+    want = orc.backproject(cfg.contents, pg.to_proj_spec_array(), chunk_for(shape, w), np.zeros(shape), yy)
+    err = rel_l2(v.as_numpy(), want)
+    print(f"fast backprojector edges ({layout}) rel L2 {err:.3e}")
+    assert (want == 0).any() and np.abs(want).max() > 0   # the field of view ends inside the volume
+    assert err <= 2e-5
+
+
 def test_backprojector_multiplier_and_16bit_volumes(device):
     import torch
 
