@@ -45,7 +45,8 @@ int32_t ap_check_launch(const char* what) {
 int32_t ap_check_array3(const ap_array3* d, const void* ptr, const char* name,
                         bool allow_f16, bool allow_bf16, bool allow_f64) {
     if (!d) return ap_fail(AP_ERR_INVALID, "%s: NULL descriptor", name);
-    if (!ptr) return ap_fail(AP_ERR_INVALID, "%s: NULL data pointer", name);
+    // an array with no elements may have no storage (as empty torch tensors do)
+    if (!ptr && array3_numel(d) != 0) return ap_fail(AP_ERR_INVALID, "%s: NULL data pointer", name);
     switch (d->dtype) {
     case AP_F32: break;
     case AP_F16: if (!allow_f16) return ap_fail(AP_ERR_UNSUPPORTED, "%s: float16 not supported here", name); break;

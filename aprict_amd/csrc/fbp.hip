@@ -150,7 +150,7 @@ int32_t check_pair(const ap_array3* sd, const void* sp, const ap_array3* ld, con
     int32_t st = ap_check_array3(sd, sp, "projections", false, false, false);
     if (st) return st;
     if ((st = ap_check_array3(ld, lp, "expanded buffer", false, false, false))) return st;
-    AP_REQUIRE(in != out, "in and out must not overlap");
+    AP_REQUIRE(in != out || array3_numel(sd) + array3_numel(ld) == 0, "in and out must not overlap");
     AP_REQUIRE(sd->shape[0] == ld->shape[0], "view counts differ");
     for (int a = 1; a < 3; a++) {
         AP_REQUIRE(ld->shape[a] >= sd->shape[a], "the expanded buffer is smaller than the projections");

@@ -85,7 +85,7 @@ extern "C" int32_t ap_ramp_filter_rows(const ap_array3* in_desc, const void* in,
     if ((st = ap_check_array3(out_desc, out, "out", false, false, false))) return st;
     for (int a = 0; a < 3; a++)
         AP_REQUIRE(in_desc->shape[a] == out_desc->shape[a], "in and out shapes differ");
-    AP_REQUIRE(in != out, "in and out must not overlap");
+    AP_REQUIRE(in != out || array3_numel(in_desc) == 0, "in and out must not overlap");
     AP_REQUIRE(physpixwid > 0.0, "physpixwid must be positive");
     AP_REQUIRE(kernel == AP_RAMP_RAMLAK || kernel == AP_RAMP_SINC || kernel == AP_RAMP_HILBERT ||
                    kernel == AP_RAMP_HILBERT_HALF,

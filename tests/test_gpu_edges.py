@@ -1,0 +1,103 @@
+"""GPU edge cases against the FP64 oracle: volumes one voxel thick along an
+axis, detectors one pixel wide or tall, calls with no views (no-ops), and
+non-finite sinogram data in the upright (column) adjoint."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from aprict_amd import _abi
+from aprict_amd import phantoms as ph
+from aprict_amd.cb import flatpanel as cbfp
+from aprict_amd.volume import Volume
+from helpers import chunk_for, rel_l2
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def to_gpu(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+CASES = {
+    "thin_x": ((1, 9, 7), (12, 10)),
+    "thin_y": ((8, 1, 6), (1, 12)),
+    "thin_z": ((7, 8, 1), (12, 1)),
+    "one_voxel_one_pixel": ((1, 1, 1), (1, 1)),
+    "odd": ((5, 3, 9), (3, 5)),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_degenerate_shapes_match_oracle(device, name):
+    shape, (nu, nv) = CASES[name]
+    rng = np.random.default_rng(len(name))
+    w = 0.7
+    # a detector wide enough that the volume's shadow falls on it in every view
+    pg = ph.circular_geometry(6, nu, nv, 1.6, R=30.0, L=60.0)
+    x = rng.uniform(0.2, 1.0, shape).astype(np.float32)
+    y = rng.uniform(-1.0, 1.0, (6, nu, nv)).astype(np.float32)
+    specs, chunk = pg.to_proj_spec_array(), chunk_for(shape, w)
+    cfg = cbfp.ConfigProject()
+    # forward
+    ps = cbfp.ProjectionSet(pg)
+    cbfp.project(cfg, Volume(x, voxwid=w), ps, block=True)
+    want = orc.project_voxel(cfg.contents, specs, chunk, x, np.zeros((6, nu, nv)))
+    assert rel_l2(ps.data.cpu().numpy(), want) <= 1e-5, "forward"
+    # matched adjoint
+    v = Volume(np.zeros(shape, np.float32), voxwid=w)
+    cbfp.backproject_matched(cfg, v, cbfp.ProjectionSet(pg, data=to_gpu(y)), block=True)
+    want = orc.backproject_matched(cfg.contents, specs, chunk, np.zeros(shape), y)
+    assert rel_l2(v.as_numpy(), want) <= 1e-5, "adjoint"
+    # fast backprojector, bilinear
+    bp = cbfp.ConfigBackproject(fast_approx=True, approx_mode_bilinear_interpolate=True)
+    v = Volume(np.zeros(shape, np.float32), voxwid=w)
+    cbfp.backproject(bp, v, cbfp.ProjectionSet(pg, data=to_gpu(y)), block=True)
+    want = orc.backproject(bp.contents, specs, chunk, np.zeros(shape), y)
+    assert rel_l2(v.as_numpy(), want) <= 2e-5, "fast backprojector"
+
+
+def test_calls_without_views_are_noops(device):
+    import torch
+
+    x = to_gpu(np.ones((4, 4, 4)))
+    sino = torch.zeros((0, 5, 5), dtype=torch.float32, device="cuda")
+    specs = torch.zeros((1, 15), dtype=torch.float32, device="cuda")   # never read with zero views
+    chunk = chunk_for((4, 4, 4), 1.0)
+    lib = _abi.lib()
+    st = _abi.stream_handle(x.device)
+    assert lib.ap_project_voxel(_abi.struct_ptr(cbfp.ConfigProject().contents), _abi.ptr(specs), 0,
+                                _abi.struct_ptr(chunk), _abi.array3(x), _abi.ptr(x), _abi.array3(sino),
+                                _abi.ptr(sino), None, st) == 0
+    assert lib.ap_backproject_matched(_abi.struct_ptr(cbfp.ConfigProject().contents), _abi.ptr(specs), 0,
+                                      _abi.struct_ptr(chunk), _abi.array3(x), _abi.ptr(x), None, None,
+                                      _abi.array3(sino), _abi.ptr(sino), 1.0, st) == 0
+    bp = cbfp.ConfigBackproject(fast_approx=True).contents
+    assert lib.ap_backproject(_abi.struct_ptr(bp), _abi.ptr(specs), 0, _abi.struct_ptr(chunk), _abi.array3(x),
+                              _abi.ptr(x), None, None, _abi.array3(sino), _abi.ptr(sino), None, None, st) == 0
+    out = torch.zeros((0, 5, 5), dtype=torch.float32, device="cuda")
+    assert lib.ap_ramp_filter_rows(_abi.array3(sino), _abi.ptr(sino), _abi.array3(out), _abi.ptr(out), 1.0, 0, 0,
+                                   st) == 0
+    torch.cuda.synchronize()
+    assert bool((x == 1).all())
+
+
+def test_column_adjoint_propagates_nan(device):
+    # upright views (the column kernel): a NaN pixel makes every voxel its ray meets NaN
+    rng = np.random.default_rng(41)
+    shape, w = (20, 18, 22), 0.6
+    pg = ph.circular_geometry(8, 30, 28, 1.0, R=30.0, L=60.0)
+    y = rng.uniform(-1.0, 1.0, (8, 30, 28)).astype(np.float32)
+    y[2, 15, 14] = np.nan
+    cfg = cbfp.ConfigProject()
+    v = Volume(np.zeros(shape, np.float32), voxwid=w)
+    cbfp.backproject_matched(cfg, v, cbfp.ProjectionSet(pg, data=to_gpu(y)), block=True)
+    got = v.as_numpy()
+    want = orc.backproject_matched(cfg.contents, pg.to_proj_spec_array(), chunk_for(shape, w), np.zeros(shape), y)
+    hit = np.isnan(want)
+    assert hit.any() and np.isnan(got[hit]).all()
+    assert np.isfinite(got).any()   # bricks whose footprints miss the pixel stay finite
