@@ -80,6 +80,7 @@ struct BpCfg {
     float w3;             // voxwid^3
     bool off32;           // detector offsets of one view fit 32 bits
     bool pair;            // v contiguous (s2 == 1), nv >= 2, 32-bit offsets: rows k, k+1 in one load
+    bool grouped;         // bp_fast_views4 applies: offsets of 4 views fit 32 bits (and pairs if bilinear)
     int zl, zv;
     uint32_t ssx, ssy, ssz;
     float w;              // voxwid
@@ -267,6 +268,97 @@ __device__ __forceinline__ float sample_det(const BpCfg& c, const ViewMap& m, co
     return y;
 }
 
+// Four consecutive views of the interpolating branch with their detector
+// reads issued together, so that a lane has four (bilinear: eight) gathers in
+// flight instead of one.  The arithmetic of each view is that of
+// bp_fast_view / sample_det, and the views are added to `acc` in order, so
+// the sums are bit-identical.  Reads use clamped in-range addresses and are
+// zeroed afterwards where the original reads nothing.  Needs 32-bit detector
+// offsets, and for bilinear taps the paired layout (c.pair).
+template <typename Tp>
+__device__ __forceinline__ void bp_fast_views4(const BpCfg& c, const ViewMap* m, const Tp* __restrict__ img,
+                                               int64_t s0, const float x[3], float& acc) {
+    float cu[4], cv[4], mult[4], ta[4], tb[4];
+    int o0[4], o1[4];
+    bool inr[4], klo[4], khi[4], jlo[4], jhi[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const ViewMap& M = m[i];
+        const float wx = M.W[0] * x[0] + M.W[1] * x[1] + M.W[2] * x[2] + M.w0;
+        const float iw = __builtin_amdgcn_rcpf(wx);
+        cu[i] = (M.Au[0] * x[0] + M.Au[1] * x[1] + M.Au[2] * x[2] + M.au) * iw;
+        cv[i] = (M.Av[0] * x[0] + M.Av[1] * x[1] + M.Av[2] * x[2] + M.av) * iw;
+        float mu = c.direct ? 1.f : M.kq * c.w3 * (iw * iw);
+        const bool pre = c.direct && c.preset != 0;
+        if (c.zl != 0 || c.zv != 0 || pre) {
+            const float zz = M.Zc[0] * x[0] + M.Zc[1] * x[1] + M.Zc[2] * x[2] + M.z0;
+            if (c.zl != 0) mu *= powi(zz / M.Lnorm, c.zl);
+            if (c.zv != 0) mu *= powi(zz / c.zval, c.zv);
+            if (pre) mu = apply_preset(c, M, x, zz, mu);
+        }
+        mult[i] = mu;
+        const float fj = cu[i] + M.hu, fk = cv[i] + M.hv;
+        const int64_t vo = (int64_t)i * s0;
+        if (c.bilinear) {
+            inr[i] = fj > -1.f && fk > -1.f && fj < (float)c.nu && fk < (float)c.nv;
+            const float fjc = inr[i] ? fj : 0.f, fkc = inr[i] ? fk : 0.f;   // finite before the conversion
+            const float fl_j = floorf(fjc), fl_k = floorf(fkc);
+            const int j0 = (int)fl_j, k0 = (int)fl_k;
+            ta[i] = fjc - fl_j;
+            tb[i] = fkc - fl_k;
+            const int ja = max(j0, 0), jb = min(j0 + 1, (int)c.nu - 1);
+            const int kc = min(max(k0, 0), (int)c.nv - 2);
+            o0[i] = (int)(vo + ja * (int32_t)c.s1 + kc);
+            o1[i] = (int)(vo + jb * (int32_t)c.s1 + kc);
+            klo[i] = k0 < 0;
+            khi[i] = k0 + 1 >= (int)c.nv;
+            jlo[i] = j0 < 0;
+            jhi[i] = j0 + 1 >= (int)c.nu;
+        } else {
+            // nearest: the index clamped to [-1, n] keeps the conversion defined
+            const int j = (int)floorf(fminf(fmaxf(fj + 0.5f, -1.f), (float)c.nu));
+            const int k = (int)floorf(fminf(fmaxf(fk + 0.5f, -1.f), (float)c.nv));
+            inr[i] = j >= 0 && k >= 0 && j < (int)c.nu && k < (int)c.nv;
+            const int jc = min(max(j, 0), (int)c.nu - 1), kk = min(max(k, 0), (int)c.nv - 1);
+            o0[i] = (int)(vo + jc * (int32_t)c.s1 + kk * (int32_t)c.s2);
+        }
+    }
+    if (c.bilinear) {
+        TapPair<Tp> pa[4], pb[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            pa[i] = *reinterpret_cast<const TapPair<Tp>*>(img + o0[i]);
+            pb[i] = *reinterpret_cast<const TapPair<Tp>*>(img + o1[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            float y = 0.f;
+            if (inr[i]) {
+                float y00 = klo[i] ? 0.f : (khi[i] ? to_f32(pa[i].hi) : to_f32(pa[i].lo));
+                float y01 = khi[i] ? 0.f : (klo[i] ? to_f32(pa[i].lo) : to_f32(pa[i].hi));
+                float y10 = klo[i] ? 0.f : (khi[i] ? to_f32(pb[i].hi) : to_f32(pb[i].lo));
+                float y11 = khi[i] ? 0.f : (klo[i] ? to_f32(pb[i].lo) : to_f32(pb[i].hi));
+                if (jlo[i]) { y00 = 0.f; y01 = 0.f; }
+                if (jhi[i]) { y10 = 0.f; y11 = 0.f; }
+                const float a = ta[i], b = tb[i];
+                const float t1 = y00 * (1.f - a) + y10 * a;
+                const float t2 = y01 * (1.f - a) + y11 * a;
+                y = t1 * (1.f - b) + t2 * b;
+            }
+            acc += mult[i] == 0.f ? 0.f : soften(c, m[i], cu[i], cv[i], y * mult[i]);
+        }
+    } else {
+        float yv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) yv[i] = to_f32(img[o0[i]]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float y = inr[i] ? yv[i] : 0.f;
+            acc += mult[i] == 0.f ? 0.f : soften(c, m[i], cu[i], cv[i], y * mult[i]);
+        }
+    }
+}
+
 template <typename Tp>
 __device__ float bp_one_view(const BpCfg& c, const ViewMap& m, const Tp* __restrict__ img,
                              const float x[3]) {
@@ -349,6 +441,12 @@ __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
             if (c.exact) {
                 for (uint32_t q = 0; q < nb; q++)
                     acc1 += bp_one_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
+            } else if (c.grouped) {
+                uint32_t q = 0;
+                for (; q + 4 <= nb; q += 4)
+                    bp_fast_views4(c, maps + q, sino + (int64_t)(p0 + q) * c.s0, c.s0, xc, acc1);
+                for (; q < nb; q++)
+                    acc1 += bp_fast_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
             } else {
                 for (uint32_t q = 0; q < nb; q++)
                     acc1 += bp_fast_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
@@ -604,6 +702,9 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
     c.off32 = (double)(c.nu ? c.nu - 1 : 0) * (double)c.s1 + (double)(c.nv ? c.nv - 1 : 0) * (double)c.s2 <
               2147483647.0;
     c.pair = c.off32 && c.s2 == 1 && c.nv >= 2;
+    c.grouped = !c.exact && (c.bilinear ? c.pair : c.off32) &&
+                (double)c.s0 * 3.0 + (double)(c.nu ? c.nu - 1 : 0) * (double)c.s1 +
+                        (double)(c.nv ? c.nv - 1 : 0) * (double)c.s2 < 2147483647.0;
     const int var = ap_tune_variant(AP_TUNE_BACKPROJECT);
     const uint32_t rows_y = (uint32_t)kBpRowsY[var];
     dim3 block(kBz, rows_y, 1);
