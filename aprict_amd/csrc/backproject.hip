@@ -34,6 +34,7 @@ struct ViewMap {
     float kq;              // K^2 / (|u| |v|): the FDK weight is kq w^3 / (W.x + w0)^2
     float hu, hv;          // (nu - 1) / 2, (nv - 1) / 2: pixel index of the detector centre
     float inv_hnu, inv_hnv;   // 2 / nu, 2 / nv
+    float inv_Lnorm;          // 1 / |D - S|
 };
 
 __device__ void make_view_map(const ap_proj_spec* specs, uint32_t p, ViewMap& m) {
@@ -57,6 +58,7 @@ __device__ void make_view_map(const ap_proj_spec* specs, uint32_t p, ViewMap& m)
     m.Zc[0] = (float)(L.x / ln); m.Zc[1] = (float)(L.y / ln); m.Zc[2] = (float)(L.z / ln);
     m.z0 = (float)(-dot(g.S, L) / ln);
     m.Lnorm = (float)ln;
+    m.inv_Lnorm = (float)(1.0 / ln);
     m.S[0] = (float)g.S.x; m.S[1] = (float)g.S.y; m.S[2] = (float)g.S.z;
     m.D[0] = (float)g.D.x; m.D[1] = (float)g.D.y; m.D[2] = (float)g.D.z;
     m.U[0] = (float)g.U.x; m.U[1] = (float)g.U.y; m.U[2] = (float)g.U.z;
@@ -76,6 +78,7 @@ struct BpCfg {
     int preset;           // backproj_weighting_preset
     float sv_a, sv_b, sh_a, sh_b;   // GBC softening: vartheta_v, its span, vartheta_h, its span
     float mask_r2, sep, zval;
+    float inv_zval;       // 1 / zval
     float inv_sep;        // 1 / sep (0 when sep == 0)
     float w3;             // voxwid^3
     bool off32;           // detector offsets of one view fit 32 bits
@@ -90,10 +93,11 @@ struct BpCfg {
     int64_t s0, s1, s2;   // sinogram strides
 };
 
+// b^e for an integer exponent: one division for negative powers, then products
 __device__ __forceinline__ float powi(float b, int e) {
+    const float f = e < 0 ? 1.f / b : b;
     float r = 1.f;
-    if (e < 0) for (int i = 0; i < -e; i++) r /= b;
-    else for (int i = 0; i < e; i++) r *= b;
+    for (int i = 0, n = e < 0 ? -e : e; i < n; i++) r *= f;
     return r;
 }
 
@@ -200,8 +204,8 @@ __device__ __forceinline__ float bp_fast_view(const BpCfg& c, const ViewMap& m, 
     const bool pre = c.direct && c.preset != 0;
     if (c.zl != 0 || c.zv != 0 || pre) {
         const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
-        if (c.zl != 0) mult *= powi(zz / m.Lnorm, c.zl);
-        if (c.zv != 0) mult *= powi(zz / c.zval, c.zv);
+        if (c.zl != 0) mult *= powi(zz * m.inv_Lnorm, c.zl);
+        if (c.zv != 0) mult *= powi(zz * c.inv_zval, c.zv);
         if (pre) mult = apply_preset(c, m, x, zz, mult);
     }
     if (mult == 0.f) return 0.f;
@@ -292,8 +296,8 @@ __device__ __forceinline__ void bp_fast_views4(const BpCfg& c, const ViewMap* m,
         const bool pre = c.direct && c.preset != 0;
         if (c.zl != 0 || c.zv != 0 || pre) {
             const float zz = M.Zc[0] * x[0] + M.Zc[1] * x[1] + M.Zc[2] * x[2] + M.z0;
-            if (c.zl != 0) mu *= powi(zz / M.Lnorm, c.zl);
-            if (c.zv != 0) mu *= powi(zz / c.zval, c.zv);
+            if (c.zl != 0) mu *= powi(zz * M.inv_Lnorm, c.zl);
+            if (c.zv != 0) mu *= powi(zz * c.inv_zval, c.zv);
             if (pre) mu = apply_preset(c, M, x, zz, mu);
         }
         mult[i] = mu;
@@ -400,11 +404,11 @@ __device__ float bp_one_view(const BpCfg& c, const ViewMap& m, const Tp* __restr
         }
         if (c.zl != 0) {
             const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
-            val *= powi(zz / m.Lnorm, c.zl);
+            val *= powi(zz * m.inv_Lnorm, c.zl);
         }
         if (c.zv != 0) {
             const float zz = m.Zc[0] * x[0] + m.Zc[1] * x[1] + m.Zc[2] * x[2] + m.z0;
-            val *= powi(zz / c.zval, c.zv);
+            val *= powi(zz * c.inv_zval, c.zv);
         }
     }
     return soften(c, m, cu, cv, val);
@@ -687,6 +691,7 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
     c.zl = cfg->Z_on_L_factor_power;
     c.zv = cfg->Z_on_val_factor_power;
     c.zval = cfg->Z_on_val_factor_val;
+    c.inv_zval = (float)(1.0 / (double)cfg->Z_on_val_factor_val);
     c.ssx = cfg->ssx; c.ssy = cfg->ssy; c.ssz = cfg->ssz;
     c.w = chunk->voxwid;
     c.c000[0] = chunk->texpos.x + 0.5f * chunk->voxwid;
