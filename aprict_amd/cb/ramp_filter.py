@@ -204,6 +204,10 @@ class FBPFilterer:
         n = v.shape[0]
         eu, ev = self._expanded_2d_shape
         dev = v.device
+        if self.hacky_fdk_mode and not np.any(angles):
+            # no rotation and no expansion: the cosine weight fused into the row filter
+            return [ramp_filter_rows(v, self.physpixwid / self.L, kernel=code, fdk_cosine_weight=True)
+                    for code in self.rfilt._codes]
         ang = torch.from_numpy(np.ascontiguousarray(angles, dtype=np.float32)).to(dev)
         buf = torch.empty((n, eu, ev), dtype=torch.float32, device=dev)
         st = _abi.stream_handle(dev)

@@ -21,7 +21,11 @@ constexpr int kMaxNu = 4096;
 __global__ __launch_bounds__(kLanes* kRows) void k_ramp_rows(
     uint32_t nu, uint32_t nv, const float* __restrict__ in, int64_t i0, int64_t i1, int64_t i2,
     float* __restrict__ out, int64_t o0, int64_t o1, int64_t o2, double p, int kernel, int fdk) {
-    __shared__ float taps[2 * kMaxNu];       // tap(n) at taps[n + nu - 1]
+    // tap(n) at taps[n + nu - 1]; valid outputs j < nu only read n = j - m in
+    // [-(nu - 1), nu - 1], and the outputs j >= nu of a block's last rows
+    // (discarded) read at most kOutPerBlock entries past 2 nu - 1: padding,
+    // so the loop needs no range check
+    __shared__ float taps[2 * kMaxNu + kOutPerBlock];
     __shared__ float xs[kChunk][kLanes];
     const int tid = threadIdx.y * kLanes + threadIdx.x;
     const uint32_t k = blockIdx.x * kLanes + threadIdx.x;
@@ -61,11 +65,7 @@ __global__ __launch_bounds__(kLanes* kRows) void k_ramp_rows(
             const float x = xs[r][threadIdx.x];
             const int base = (int)jb - (int)(m0 + r) + (int)nu - 1;
 #pragma unroll
-            for (int q = 0; q < kPerThread; q++) {
-                const int ti = base + q;
-                const float t = (ti >= 0 && ti < (int)(2 * nu - 1)) ? taps[ti] : 0.f;
-                acc[q] = __builtin_fmaf(x, t, acc[q]);
-            }
+            for (int q = 0; q < kPerThread; q++) acc[q] = __builtin_fmaf(x, taps[base + q], acc[q]);
         }
     }
     if (k >= nv) return;
