@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kWaveV* kRowsU) void k_project_voxel(
 }
 
 // ---------------------------------------------------------------- adjoint
-constexpr int kBX = 32, kBY = 32, kBZ = 8;     // brick (voxels), z fastest in LDS
+constexpr int kBX = 16, kBY = 16, kBZ = 16;    // brick (voxels), z fastest in LDS
 constexpr int kAdjThreads = 256;
 constexpr int kViewBatchAdj = 32;
 
@@ -155,7 +155,8 @@ __device__ __forceinline__ bool project_point(const ViewRays& vd, const float X[
 }
 
 template <typename Tv, typename Tp, typename Tm, typename IdxT>
-__global__ __launch_bounds__(kAdjThreads) void k_backproject_matched(
+// 4 waves per SIMD (at most 128 VGPRs): the 32 KB brick leaves room for 4 workgroups per CU
+__global__ __launch_bounds__(kAdjThreads, 4) void k_backproject_matched(
     RayCfg rc, SinoCfg sc, float scale, uint32_t ssu, uint32_t ssv,
     const ap_proj_spec* __restrict__ specs, Tv* __restrict__ vol, IdxT vsx, IdxT vsy, IdxT vsz,
     const Tm* __restrict__ mult, IdxT msx, IdxT msy, IdxT msz, const Tp* __restrict__ sino,
