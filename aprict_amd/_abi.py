@@ -34,6 +34,7 @@ AP_FBP_GBC, AP_FBP_FAN, AP_FBP_FDK = 0, 1, 2
 # --------------------------------------------------------------------------
 fvec3 = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4")], align=True)
 uivec3 = np.dtype([("x", "<u4"), ("y", "<u4"), ("z", "<u4")], align=True)
+ulvec3 = np.dtype([("x", "<u8"), ("y", "<u8"), ("z", "<u8")], align=True)
 det_spec = np.dtype(
     [("pos", fvec3), ("u", fvec3), ("v", fvec3), ("nu", "<u4"), ("nv", "<u4"), ("type", "u1")],
     align=True,
@@ -105,6 +106,12 @@ forbild_shape = np.dtype(
     align=True,
 )
 
+# the reference's by-value argument of one backprojection launch (10 views)
+proj_chunk_spec_maxlen10 = np.dtype(
+    [("projspecs", proj_spec, (10,)), ("numprojs", "<u1"), ("projmemaddr", "<u8"), ("projmemstrides", ulvec3)],
+    align=True,
+)
+
 _EXPECTED_SIZES = {
     0: det_spec.itemsize,
     1: proj_spec.itemsize,
@@ -115,6 +122,7 @@ _EXPECTED_SIZES = {
     6: forbild_clip.itemsize,
     7: forbild_shape.itemsize,
     8: 40,
+    9: proj_chunk_spec_maxlen10.itemsize,
 }
 assert _EXPECTED_SIZES[0] == 48 and _EXPECTED_SIZES[1] == 60, "numpy det/proj spec layout"
 assert _EXPECTED_SIZES[5] == 60 and _EXPECTED_SIZES[7] == 156, "numpy config/shape layout"

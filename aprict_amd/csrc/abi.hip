@@ -24,6 +24,10 @@ static_assert(sizeof(ap_forbild_shape) == 156, "FORBILD_Shape layout");
 static_assert(offsetof(ap_forbild_shape, x) == 68, "FORBILD_Shape.x offset");
 static_assert(offsetof(ap_forbild_shape, a_x) == 108, "FORBILD_Shape.a_x offset");
 static_assert(sizeof(ap_array3) == 40, "ap_array3 layout");
+static_assert(sizeof(ap_proj_chunk_spec_maxlen10) == 640, "proj_chunk_spec_maxlen10 layout");
+static_assert(offsetof(ap_proj_chunk_spec_maxlen10, numprojs) == 600, "chunk10.numprojs offset");
+static_assert(offsetof(ap_proj_chunk_spec_maxlen10, projmemaddr) == 608, "chunk10.projmemaddr offset");
+static_assert(offsetof(ap_proj_chunk_spec_maxlen10, projmemstrides) == 616, "chunk10.projmemstrides offset");
 
 static thread_local char g_last_error[512] = "no error";
 
@@ -98,6 +102,7 @@ int32_t ap_struct_size(int32_t which) {
     case 6: return (int32_t)sizeof(ap_forbild_clip);
     case 7: return (int32_t)sizeof(ap_forbild_shape);
     case 8: return (int32_t)sizeof(ap_array3);
+    case 9: return (int32_t)sizeof(ap_proj_chunk_spec_maxlen10);
     default: return -1;
     }
 }

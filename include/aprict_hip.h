@@ -54,6 +54,7 @@ extern "C" {
 /* ---- reference kernel-argument structs --------------------------------- */
 typedef struct { float x, y, z; } ap_fvec3;
 typedef struct { uint32_t x, y, z; } ap_uivec3;
+typedef struct { uint64_t x, y, z; } ap_ulvec3;
 
 /* det_spec: cuda/structs/geometry.py:40-53 (48 bytes) */
 typedef struct {
@@ -70,6 +71,21 @@ typedef struct {
     ap_fvec3    sourcepos;
     ap_det_spec dspec;
 } ap_proj_spec;
+
+/* proj_chunk_spec_maxlen10: cuda/structs/geometry.py:112-123 (640 bytes).
+ * The by-value argument of one reference backprojection launch
+ * (CB_backproject.cu:952-965): up to 10 views, the device address of the
+ * first view's projection data and its byte strides (view, u, v).  The
+ * entry points here take the views as a device array instead and need no
+ * 10-view chunks; the type is defined so that a caller holding these
+ * structs can map them: specs = projspecs[0..numprojs), data at
+ * projmemaddr with element strides projmemstrides / element size.        */
+typedef struct {
+    ap_proj_spec projspecs[10];
+    uint8_t      numprojs;
+    uint64_t     projmemaddr;
+    ap_ulvec3    projmemstrides;
+} ap_proj_chunk_spec_maxlen10;
 
 /* vol_tex_chunk_spec: cuda/structs/geometry.py:175-186 (40 bytes).
  * texpos = corner of voxel (0,0,0) of the whole volume; offsets/shape

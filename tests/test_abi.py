@@ -34,10 +34,15 @@ def test_library_exports_every_declared_function():
 
 def test_struct_layouts_match_reference_sizes():
     lib = aprict_amd.lib()
-    want = {0: 48, 1: 60, 2: 40, 3: 16, 4: 16, 5: 60, 6: 20, 7: 156, 8: 40}
+    want = {0: 48, 1: 60, 2: 40, 3: 16, 4: 16, 5: 60, 6: 20, 7: 156, 8: 40, 9: 640}
     for which, size in want.items():
         assert lib.ap_struct_size(which) == size
     assert lib.ap_struct_size(99) == -1
+    # proj_chunk_spec_maxlen10 (the reference's 10-view launch argument): 10 x 60 B of
+    # specs, the view count at 600, the data address at 608, byte strides at 616
+    c10 = _abi.proj_chunk_spec_maxlen10
+    assert c10.itemsize == 640
+    assert [c10.fields[f][1] for f in ("projspecs", "numprojs", "projmemaddr", "projmemstrides")] == [0, 600, 608, 616]
     assert _abi.forbild_shape.fields["x"][1] == 68
     assert _abi.proj_spec.fields["dspec"][1] == 12
     assert _abi.config_cb_backproject.fields["ssx"][1] == 32
