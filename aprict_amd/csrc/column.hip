@@ -29,7 +29,15 @@ struct SinoCfgC {
     int64_t s0, s1, s2;
     uint32_t vec16;    // f32 or f16, v contiguous, rows 16-byte aligned, nv a multiple of 16 bytes
     float unit;        // adjoint fixed-point unit 2^b (adjoint_fixed_point_unit)
+    uint32_t o32;      // every element offset fits 31 bits: 32-bit address arithmetic
 };
+
+// element offset of sinogram sample (p, j, k)
+__device__ __forceinline__ int64_t sino_off(const SinoCfgC& sc, uint32_t p, int j, int k) {
+    if (sc.o32)
+        return (int64_t)(uint32_t)((int32_t)p * (int32_t)sc.s0 + j * (int32_t)sc.s1 + k * (int32_t)sc.s2);
+    return (int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2;
+}
 
 struct ColMap {
     uint32_t nkb;      // row blocks per column
@@ -483,7 +491,7 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
             for (int e = lane; e < ngrp; e += 64) {
                 const int jj = udiv(e, dg);
                 const uint4 raw = *reinterpret_cast<const uint4*>(
-                    sino + (int64_t)p * sc.s0 + (int64_t)(j0 + jj) * sc.s1 + ka + kVec * (e - jj * ng));
+                    sino + sino_off(sc, p, j0 + jj, ka + kVec * (e - jj * ng)));
                 const Tp* v = reinterpret_cast<const Tp*>(&raw);
                 float m = 0.f, sum = 0.f;
 #pragma unroll
@@ -501,8 +509,7 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
         const UDiv dk = make_udiv(nk);
         for (int e = lane; e < npix; e += 64) {
             const int jj = udiv(e, dk);
-            const float y = to_f32(sino[(int64_t)p * sc.s0 + (int64_t)(j0 + jj) * sc.s1 +
-                                        (int64_t)(k0 + e - jj * nk) * sc.s2]);
+            const float y = to_f32(sino[sino_off(sc, p, j0 + jj, k0 + e - jj * nk)]);
             ymax = fmaxf(ymax, fabsf(y));   // a NaN sample is caught below
             if (y != y) ymax = __builtin_inff();
         }
@@ -609,7 +616,7 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                 const int e = min(e0, npair - 1);
                 const int g = udiv(e, drow);
                 const int k = k0 + (ss1 ? e - g * nrow : udiv(e - g * nrow, dsv));
-                return to_f32(sino[(int64_t)p * sc.s0 + (int64_t)W.ce[g].j * sc.s1 + (int64_t)k * sc.s2]);
+                return to_f32(sino[sino_off(sc, p, W.ce[g].j, k)]);
             };
             auto pair = [&](int e, float y) {
                 const int g = udiv(e, drow), kk = e - g * nrow;
@@ -813,6 +820,8 @@ int32_t launch_backproject_columns(const RayCfg& rc, uint32_t V, uint32_t nu, ui
                                    int32_t mdt, const int64_t* ms, const void* mult, int32_t sino_dtype,
                                    const void* sino, bool idx64, hipStream_t s) {
     SinoCfgC sc{V, nu, nv, s0, s1, s2, 0u};
+    sc.o32 = (double)(V ? V - 1 : 0) * (double)s0 + (double)(nu ? nu - 1 : 0) * (double)s1 +
+                     (double)(nv ? nv - 1 : 0) * (double)s2 < 2147483647.0;
     {
         const int64_t kv = sino_dtype == AP_F32 ? 4 : sino_dtype == AP_F16 ? 8 : 0;
         sc.vec16 = kv > 0 && s2 == 1 && nv % kv == 0 && s1 % kv == 0 && s0 % kv == 0 &&
