@@ -97,3 +97,16 @@ def test_projection_set_validation():
     ps.create_buffer(dtype=np.float32, existing_ok=True)
     z = ps.zeroed_copy()
     assert z.data.shape == ps.data.shape
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # bench.py measures one GPU; --gpus N > 1 exits with a message before touching a device
+    import pathlib
+    import subprocess
+    import sys
+
+    root = pathlib.Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, check=False)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
