@@ -35,6 +35,7 @@ struct Prim {
     float rho;
     int cls;
     float bc[3], br2;   // bounding sphere (centre, squared radius) for culling; br2 NaN: never culled
+    float br;           // its radius
 };
 
 // Radius of a sphere about c holding the primitive: |M^-1 l| <= sigma_max(M^-1) |l|
@@ -62,6 +63,7 @@ __device__ void set_bound(Prim& P, double lr) {
     P.bc[1] = (float)P.c[1];
     P.bc[2] = (float)P.c[2];
     P.br2 = (float)(r * r);   // inf or NaN (degenerate shape) never culls
+    P.br = (float)r;
 }
 
 __device__ __forceinline__ d3 unit_of(ap_fvec3 a) {
@@ -482,28 +484,50 @@ __global__ __launch_bounds__(256) void k_voxelise(
     // per sub-point value of the last containing shape; ss <= 4 => <= 64 points
     float val[64];
     for (uint32_t q = 0; q < npt && q < 64; q++) val[q] = 0.f;
+    float val1 = 0.f;   // the one point of ss == 1, kept in a register
+    // voxel centre in FP32 for the bounding-sphere cull; every sub-point lies
+    // within the voxel's half-diagonal of it
+    const float cf[3] = {(float)(tpx + (gi + 0.5) * vw), (float)(tpy + (gj + 0.5) * vw),
+                         (float)(tpz + (gk + 0.5) * vw)};
+    const float reach = (float)(0.8660254037844386 * vw);
     for (uint32_t k0 = 0; k0 < K; k0 += kShapeBatch) {
         const uint32_t nb = min((uint32_t)kShapeBatch, K - k0);
         __syncthreads();
         if (tid < (int)nb) make_prim(shapes[k0 + tid], prims[tid]);
         __syncthreads();
         if (!in) continue;
-        for (uint32_t a = 0, q = 0; a < ss; a++)
-            for (uint32_t b = 0; b < ss; b++)
-                for (uint32_t e = 0; e < ss; e++, q++) {
-                    const double fx = ((double)a - 0.5 * (ss - 1.0)) / ss;
-                    const double fy = ((double)b - 0.5 * (ss - 1.0)) / ss;
-                    const double fz = ((double)e - 0.5 * (ss - 1.0)) / ss;
-                    const d3 pnt = mkd3(tpx + (gi + 0.5 + fx) * vw, tpy + (gj + 0.5 + fy) * vw,
-                                        tpz + (gk + 0.5 + fz) * vw);
-                    for (uint32_t s = 0; s < nb; s++)
-                        if (prim_contains(prims[s], pnt)) val[q] = prims[s].rho;
-                }
+        for (uint32_t s = 0; s < nb; s++) {
+            const Prim& P = prims[s];
+            // outside the bounding sphere grown by the half-diagonal, with a
+            // margin for the FP32 centre (shapes are visited in order, so the
+            // last containing shape still wins)
+            const float dx = cf[0] - P.bc[0], dy = cf[1] - P.bc[1], dz = cf[2] - P.bc[2];
+            const float rr = P.br * 1.0001f + reach + 1e-3f;
+            if (dx * dx + dy * dy + dz * dz > rr * rr) continue;
+            if (ss == 1) {
+                if (prim_contains(P, mkd3(tpx + (gi + 0.5) * vw, tpy + (gj + 0.5) * vw, tpz + (gk + 0.5) * vw)))
+                    val1 = P.rho;
+                continue;
+            }
+            for (uint32_t a = 0, q = 0; a < ss; a++)
+                for (uint32_t b = 0; b < ss; b++)
+                    for (uint32_t e = 0; e < ss; e++, q++) {
+                        const double fx = ((double)a - 0.5 * (ss - 1.0)) / ss;
+                        const double fy = ((double)b - 0.5 * (ss - 1.0)) / ss;
+                        const double fz = ((double)e - 0.5 * (ss - 1.0)) / ss;
+                        const d3 pnt = mkd3(tpx + (gi + 0.5 + fx) * vw, tpy + (gj + 0.5 + fy) * vw,
+                                            tpz + (gk + 0.5 + fz) * vw);
+                        if (prim_contains(P, pnt)) val[q] = P.rho;
+                    }
+        }
     }
     if (!in) return;
-    float tot = 0.f;
-    for (uint32_t q = 0; q < npt; q++) tot += val[q];
-    tot /= (float)npt;
+    float tot = val1;
+    if (ss != 1) {
+        tot = 0.f;
+        for (uint32_t q = 0; q < npt; q++) tot += val[q];
+        tot /= (float)npt;
+    }
     const int64_t o = (int64_t)gi * v0 + (int64_t)gj * v1 + (int64_t)gk * v2;
     vol[o] = from_f32<Tv>(to_f32(vol[o]) + tot);
 }
