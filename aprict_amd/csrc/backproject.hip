@@ -9,6 +9,8 @@
 // One thread per voxel; a wavefront covers 64 consecutive voxels along the
 // volume's z axis.  Views are processed in batches of kViewBatch: the block
 // derives each view's projective map once (FP64, rounded to FP32) into LDS.
+#include <cmath>
+
 #include "ap_common.h"
 
 namespace {
@@ -73,10 +75,47 @@ __device__ void make_view_map(const ap_proj_spec* specs, uint32_t p, ViewMap& m)
     m.inv_hnv = (float)(2.0 / (double)g.nv);
 }
 
+// cos^2-type softening window s(a, b; t) of the GBC paper's appendix
+// (CB_backproject.cu:274-311), 1 for |t| < a - b, 0 for |t| > a and
+// 3x^2 - 2x^3 between with x = (sin|t| - sin a) / (sin(a - b) - sin a), held as
+// constants of sin|t|.  Both GBC angles (the elevation and the horizontal
+// angle) lie in [0, pi/2], where sine increases, so |t| > a <=> sin|t| > hi
+// and |t| < a - b <=> sin|t| < lo, with hi = sin a (2 when a >= pi/2: never
+// exceeded, -1 when a < 0: always) and lo = sin(a - b) (-1 when a - b <= 0:
+// never undercut, 2 when a - b > pi/2: always); g = 1 / (sin(a - b) - sin a)
+// and gs = g sin a.
+struct SoftWin {
+    float hi, lo, g, gs;
+};
+
+// GBC weighting constants: 1 / R (R the radius of the source cylinder about
+// the z axis) and the windows of the elevation (vartheta_v) and of the
+// horizontal angle (vartheta_h)
+struct GbcSoft {
+    float inv_R;
+    SoftWin v, h;
+};
+
+inline SoftWin make_soft_win(float a32, float b32) {
+    const double a = a32, d = (double)a32 - (double)b32, hp = 1.5707963267948966;
+    SoftWin s;
+    s.hi = a < 0.0 ? -1.f : (a >= hp ? 2.f : (float)std::sin(a));
+    s.lo = d <= 0.0 ? -1.f : (d > hp ? 2.f : (float)std::sin(d));
+    const double g = 1.0 / (std::sin(d) - std::sin(a));
+    s.g = (float)g;
+    s.gs = (float)(g * std::sin(a));
+    return s;
+}
+
+inline GbcSoft make_gbc_soft(const ap_config_cb_backproject* cfg, double R) {
+    return GbcSoft{(float)(1.0 / R), make_soft_win(cfg->soft_vartheta_v, cfg->soft_vartheta_v_softspan),
+                   make_soft_win(cfg->soft_vartheta_h, cfg->soft_vartheta_h_softspan)};
+}
+
 struct BpCfg {
     bool exact, all1s, mask, direct, bilinear, use_mult;
     int preset;           // backproj_weighting_preset
-    float sv_a, sv_b, sh_a, sh_b;   // GBC softening: vartheta_v, its span, vartheta_h, its span
+    GbcSoft gbc;          // GBC weighting (preset 7)
     float mask_r2, sep, zval;
     float inv_zval;       // 1 / zval
     float inv_sep;        // 1 / sep (0 when sep == 0)
@@ -490,49 +529,42 @@ __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
     vol[o] = from_f32<Tv>(to_f32(vol[o]) + acc1 * mval);
 }
 
-// cos^2-type softening window s(a, b; c) of the GBC paper's appendix
-// (CB_backproject.cu:274-311): 1 for |c| < a - b, 0 for |c| > a, and
-// 3x^2 - 2x^3 in between with x = (sin|c| - sin a) / (sin(a - b) - sin a).
-__device__ __forceinline__ float cossq_soft(float a, float b, float c) {
-    const float ac = fabsf(c);
-    if (ac > a) return 0.f;
-    if (ac < a - b) return 1.f;
-    const float g = 1.f / (sinf(a - b) - sinf(a));
-    const float x = g * sinf(ac) - g * sinf(a);
+// the window s at sn = sin|t|
+__device__ __forceinline__ float soft_win(const SoftWin& s, float sn) {
+    if (sn > s.hi) return 0.f;
+    if (sn < s.lo) return 1.f;
+    const float x = s.g * sn - s.gs;
     return fmaxf(0.f, fminf(1.f, 3.f * x * x - 2.f * x * x * x));
 }
 
-// GBC weight of one view for a source on a cylinder of radius R about the z
+// GBC weight of one view for a source S on a cylinder of radius R about the z
 // axis (CB_backproject.cu:736-790, eq. (23) of the GBC paper): with ell the
-// horizontal source->voxel distance, c the |cosine| of the horizontal angle
-// between that direction and the direction to the axis (both over R) and
-// theta the elevation of the voxel seen from the source,
+// horizontal source->voxel distance over R, c the |cosine| of the horizontal
+// angle between that direction and the direction to the axis, and theta the
+// elevation of the voxel seen from the source,
 //   w = cos^3(theta) c / (c^2 + (c - ell)^2) * s(v; theta) * s(h; acos c).
-struct GbcSoft {
-    float R;                   // source radius (the cylinder's axis is the z axis)
-    float va, vb, ha, hb;      // soft_vartheta_v, its span, soft_vartheta_h, its span
-};
-
+// Trigonometry by square roots: with r = x - S, cos theta = |r_xy| / |r|,
+// sin|theta| = |r_z| / |r|, c = |r_xy . S_xy| / (|r_xy| |S_xy|) and
+// sin(acos c) = sqrt(1 - c^2).  A point straight above or below the source
+// (r_xy = 0) gets c = 1 and cos theta = 0, as the angle form gives.
 __device__ __forceinline__ float gbc_weight(const GbcSoft& g, const float S[3], const float x[3]) {
-    const float R = g.R;
-    const float lx = -S[0], ly = -S[1];
     const float r0 = x[0] - S[0], r1 = x[1] - S[1], r2 = x[2] - S[2];
-    const float zz = (r0 * lx + r1 * ly) / sqrtf(lx * lx + ly * ly);
-    const float ell = sqrtf(r0 * r0 + r1 * r1) / R;
-    const float zr = zz / R;
-    const float cr = fmaxf(0.f, fminf(1.f, fabsf(zr / ell)));
-    const float dr = cr - ell;
-    const float th = atan2f(r2 / R, ell);
-    const float ce = cosf(th);
+    const float rxy2 = r0 * r0 + r1 * r1;
+    const float rxy = sqrtf(rxy2);
+    const float ir = rsqrtf(rxy2 + r2 * r2);
+    const float il = rsqrtf(S[0] * S[0] + S[1] * S[1]);
+    const float cr = fminf(1.f, fabsf(r0 * S[0] + r1 * S[1]) * il * rsqrtf(rxy2));
+    const float dr = cr - rxy * g.inv_R;
+    const float ce = rxy * ir;
     const float w = ce * ce * ce * cr / fmaxf(1e-9f, cr * cr + dr * dr);
-    return w * (cossq_soft(g.va, g.vb, th) * cossq_soft(g.ha, g.hb, acosf(cr)));
+    return w * (soft_win(g.v, fabsf(r2) * ir) * soft_win(g.h, sqrtf(fmaxf(0.f, 1.f - cr * cr))));
 }
 
 // one view's GBC contribution: value += w * detector(x), weight += w
 template <typename Tp>
 __device__ __forceinline__ void gbc_view(const BpCfg& c, const ViewMap& m, const Tp* __restrict__ img,
                                          const float x[3], float& val, float& wsum) {
-    const float w = gbc_weight(GbcSoft{c.zval, c.sv_a, c.sv_b, c.sh_a, c.sh_b}, m.S, x);
+    const float w = gbc_weight(c.gbc, m.S, x);
     wsum += w;
     if (w == 0.f) return;
     const float iw = __builtin_amdgcn_rcpf(m.W[0] * x[0] + m.W[1] * x[1] + m.W[2] * x[2] + m.w0);
@@ -621,13 +653,14 @@ __global__ __launch_bounds__(kBz* kGbcRows) void k_backproject_gbc(
 // FP64 and adds the sum, times the quadrature weight, to out[r].
 constexpr int kIntegThreads = 256, kIntegAngles = 16;
 
-__global__ __launch_bounds__(kIntegThreads) void k_gbc_weight_integ(GbcSoft g, const float* __restrict__ r_on_R,
-                                                                    uint32_t n, double* __restrict__ out,
-                                                                    uint32_t nz, uint32_t nt) {
+__global__ __launch_bounds__(kIntegThreads) void k_gbc_weight_integ(GbcSoft g, float tan_va,
+                                                                    const float* __restrict__ r_on_R, uint32_t n,
+                                                                    double* __restrict__ out, uint32_t nz,
+                                                                    uint32_t nt) {
     __shared__ double red[kIntegThreads / 64];
     const uint32_t ri = blockIdx.y;
     const float x[3] = {r_on_R[ri], 0.f, 0.f};
-    const float maxz = tanf(g.va) * (1.f + r_on_R[n - 1]);
+    const float maxz = tan_va * (1.f + r_on_R[n - 1]);
     const uint32_t a0 = blockIdx.x * kIntegAngles;
     const uint32_t na = min((uint32_t)kIntegAngles, nt - a0);
     double acc = 0.0;
@@ -660,11 +693,11 @@ extern "C" int32_t ap_gbc_weight_integ(const float* r_on_R, uint32_t n, double* 
     AP_REQUIRE(num_z_samps >= 2, "num_z_samps must be >= 2");
     AP_REQUIRE(num_ang_samps >= 1, "num_ang_samps must be >= 1");
     AP_REQUIRE(n <= 65535u, "at most 65535 radii per call");
-    const GbcSoft g{1.f, cfg->soft_vartheta_v, cfg->soft_vartheta_v_softspan, cfg->soft_vartheta_h,
-                    cfg->soft_vartheta_h_softspan};
+    const GbcSoft g = make_gbc_soft(cfg, 1.0);
+    const float tan_va = (float)std::tan((double)cfg->soft_vartheta_v);
     dim3 grid((num_ang_samps + kIntegAngles - 1) / kIntegAngles, n, 1);
-    hipLaunchKernelGGL(k_gbc_weight_integ, grid, dim3(kIntegThreads), 0, (hipStream_t)stream, g, r_on_R, n, out,
-                       num_z_samps, num_ang_samps);
+    hipLaunchKernelGGL(k_gbc_weight_integ, grid, dim3(kIntegThreads), 0, (hipStream_t)stream, g, tan_va, r_on_R, n,
+                       out, num_z_samps, num_ang_samps);
     return ap_check_launch("gbc_weight_integ");
 }
 
@@ -676,10 +709,7 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
                                  const ap_array3* sec_desc, void* secondary, hipStream_t s) {
     BpCfg c;
     c.preset = cfg->backproj_weighting_preset;
-    c.sv_a = cfg->soft_vartheta_v;
-    c.sv_b = cfg->soft_vartheta_v_softspan;
-    c.sh_a = cfg->soft_vartheta_h;
-    c.sh_b = cfg->soft_vartheta_h_softspan;
+    c.gbc = make_gbc_soft(cfg, cfg->Z_on_val_factor_val);
     c.exact = cfg->exact_line_integ_backproj != 0;
     c.all1s = cfg->backproject_all_1s != 0;
     c.mask = cfg->mask_to_cylinder != 0;
