@@ -23,8 +23,9 @@
 // Per-column constants of the row z walks (FP64).
 struct ColZ {
     double hc;                 // closest-approach parameter of the 2-D line
-    double il;                 // 1 / horizontal source-pixel distance
-    double sz, qz0, vz;        // source z, pixel z at kv = 0, z step per row
+    double m0, dm;             // slope dz/dh of row kv: m0 + kv dm
+    double zc;                 // source z plus the chunk's z centre (index units); lane_setup
+                               // takes the z range [lo, hi) col_setup was given
 };
 
 // The shared 2-D line of one detector (sub-)column.
@@ -57,10 +58,9 @@ __device__ inline bool col_setup(const ViewRays& vr, const int lo[3], const int 
     const double u[2] = {dx * il, dy * il};
     const double hc = -(vr.S[0] * u[0] + vr.S[1] * u[1]);
     C.z.hc = hc;
-    C.z.il = il;
-    C.z.sz = vr.S[2];
-    C.z.qz0 = vr.Q00[2] + (double)ju * vr.U[2];
-    C.z.vz = vr.V[2];
+    C.z.m0 = (vr.Q00[2] + (double)ju * vr.U[2] - vr.S[2]) * il;
+    C.z.dm = vr.V[2] * il;
+    C.z.zc = vr.S[2] + 0.5 * (double)(lo[2] + hi[2]);
     C.hs = (float)(-hc);
     C.hd = (float)(l - hc);
     for (int a = 0; a < 2; a++) {
@@ -184,9 +184,8 @@ __device__ __forceinline__ int seq_count(const AxisSeq& q, float h, bool strict,
 // constants.  Returns false for a ray that cannot meet [zlo, zhi)
 // (horizontal rays outside it).
 __device__ inline bool lane_setup(const ColZ& C, float kv, int zlo, int zhi, LaneZ& Z) {
-    const double qz = C.qz0 + (double)kv * C.vz;
-    const double m = (qz - C.sz) * C.il;
-    const double zab = C.sz + C.hc * m + 0.5 * (double)(zlo + zhi);
+    const double m = fma((double)kv, C.dm, C.m0);
+    const double zab = fma(C.hc, m, C.zc);
     const float mf = (float)m;
     Z.sec = __builtin_amdgcn_sqrtf(__builtin_fmaf(mf, mf, 1.f));
     Z.u = mf;

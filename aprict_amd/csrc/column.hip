@@ -639,9 +639,6 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                 const float wys = y * scaleS * Z.sec;
                 const float* he = W.he[g];
                 const int* ob = W.ob[g];
-                // first segment ending after t0 (he[nseg-1] = h1 > t0): a binary
-                // search, a few dependent LDS reads instead of one per segment
-                const int s0 = first_above(he, E.nseg, t0);
                 float hp = t0;
                 if (Z.u > -kSingleSlope && Z.u < kSingleSlope) {
                     // at most one z crossing per segment: 4 segments per step
@@ -653,9 +650,16 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                     // issue, so its wait does not include them (LDS returns in
                     // order); chunks start at most at kListCap - 8, so the read
                     // ahead stays inside the list; the padding ends the walk
-                    // with the chunk whose last entry reaches t1
+                    // with the chunk whose last entry reaches t1.  The walk
+                    // starts at segment 0: segments that end before t0 clamp
+                    // to [t0, t0], weight 0 and hold no z crossing (the next
+                    // one, hz, lies beyond t0), so they add zero terms (one
+                    // rounding unit for a negative ray) to cells of the brick;
+                    // most rows enter the brick at h0, where the walk starts
+                    // anyway, and a search for the first segment would cost
+                    // more than the few extra chunks of the others
                     const FxRay R = fx_ray(wys);
-                    int s = s0 & ~3;
+                    int s = 0;
                     float4 h4 = *reinterpret_cast<const float4*>(he + s);
                     int4 o4 = *reinterpret_cast<const int4*>(ob + s);
                     // complete the first read here, so that the loop head
@@ -709,6 +713,8 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                         s += 4;
                     }
                 } else {
+                    // first segment ending after t0 (he[nseg-1] = h1 > t0)
+                    const int s0 = first_above(he, E.nseg, t0);
                     for (int s = s0; s < E.nseg; s++) {
                         const float cend = __builtin_amdgcn_fmed3f(he[s], t0, t1);
                         const int o = ob[s] / 8;
