@@ -69,6 +69,28 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// Minimum / maximum over lanes 0-7 of a wave (all lanes active), as a
+// wave-uniform value: quad permutations [1,0,3,2] and [2,3,0,1] and a mirror
+// within each half row of 8 lanes (DPP moves, no LDS), then lane 0's value.
+template <int kCtrl>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float lanes8_min(float v) {
+    v = fminf(v, dpp_mov<0xB1>(v));
+    v = fminf(v, dpp_mov<0x4E>(v));
+    v = fminf(v, dpp_mov<0x141>(v));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ __forceinline__ float lanes8_max(float v) {
+    v = fmaxf(v, dpp_mov<0xB1>(v));
+    v = fmaxf(v, dpp_mov<0x4E>(v));
+    v = fmaxf(v, dpp_mov<0x141>(v));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // First index s in [0, n) with he[s] > x (n if none); he is non-decreasing.
 __device__ __forceinline__ int first_above(const float* he, int n, float x) {
     int a = 0, b = n;
@@ -340,6 +362,7 @@ constexpr int kTX = 16, kTY = 16, kTZ = 32;
 constexpr int kAdjT = 1024;
 constexpr int kAdjWaves = kAdjT / 64;
 constexpr int kWCols = 8;               // sub-columns per wave pass
+constexpr int kPass1Loads = 8;          // sinogram loads per lane in flight in the scale pass
 // segments of one column inside a brick (<= kTX + kTY - 1) plus 3 padding
 // entries, rounded to whole float4 rows
 constexpr int kListCap = kTX + kTY + 4;
@@ -416,9 +439,9 @@ __device__ inline bool brick_footprint(const ViewRays& vr, const int blo[3], con
             bad = 1.f;
         }
     }
-    jmn = wave_min(jmn); jmx = wave_max(jmx);
-    kmn = wave_min(kmn); kmx = wave_max(kmx);
-    if (wave_max(bad) > 0.f) {
+    jmn = lanes8_min(jmn); jmx = lanes8_max(jmx);
+    kmn = lanes8_min(kmn); kmx = lanes8_max(kmx);
+    if (lanes8_max(bad) > 0.f) {
         j0 = 0; j1 = (int)nu - 1; k0 = 0; k1 = (int)nv - 1;
     } else {
         j0 = max(0, (int)floorf(jmn - 0.501f));
@@ -488,30 +511,49 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
             const int ka = k0 & ~(kVec - 1), ng = ((k1 | (kVec - 1)) - ka + 1) / kVec;
             const int ngrp = (j1 - j0 + 1) * ng;
             const UDiv dg = make_udiv(ng);
-            for (int e = lane; e < ngrp; e += 64) {
-                const int jj = udiv(e, dg);
-                const uint4 raw = *reinterpret_cast<const uint4*>(
-                    sino + sino_off(sc, p, j0 + jj, ka + kVec * (e - jj * ng)));
-                const Tp* v = reinterpret_cast<const Tp*>(&raw);
-                float m = 0.f, sum = 0.f;
+            // kPass1Loads loads per lane in flight before any is reduced;
+            // indices past the footprint are clamped to its last group (a
+            // repeated group leaves the maximum unchanged)
+            for (int e0 = lane; e0 < ngrp; e0 += 64 * kPass1Loads) {
+                uint4 raw[kPass1Loads];
 #pragma unroll
-                for (int q = 0; q < kVec; q++) {
-                    const float y = to_f32(v[q]);
-                    m = fmaxf(m, fabsf(y));
-                    sum += y;
+                for (int u = 0; u < kPass1Loads; u++) {
+                    const int e = min(e0 + 64 * u, ngrp - 1);
+                    const int jj = udiv(e, dg);
+                    raw[u] = *reinterpret_cast<const uint4*>(
+                        sino + sino_off(sc, p, j0 + jj, ka + kVec * (e - jj * ng)));
                 }
-                ymax = fmaxf(ymax, m);
-                if (sum != sum) ymax = __builtin_inff();   // NaN (or opposite infinities)
+#pragma unroll
+                for (int u = 0; u < kPass1Loads; u++) {
+                    const Tp* v = reinterpret_cast<const Tp*>(&raw[u]);
+                    float m = 0.f, sum = 0.f;
+#pragma unroll
+                    for (int q = 0; q < kVec; q++) {
+                        const float y = to_f32(v[q]);
+                        m = fmaxf(m, fabsf(y));
+                        sum += y;
+                    }
+                    ymax = fmaxf(ymax, m);
+                    if (sum != sum) ymax = __builtin_inff();   // NaN (or opposite infinities)
+                }
             }
             continue;
         }
         const int nk = k1 - k0 + 1, npix = (j1 - j0 + 1) * nk;
         const UDiv dk = make_udiv(nk);
-        for (int e = lane; e < npix; e += 64) {
-            const int jj = udiv(e, dk);
-            const float y = to_f32(sino[sino_off(sc, p, j0 + jj, k0 + e - jj * nk)]);
-            ymax = fmaxf(ymax, fabsf(y));   // a NaN sample is caught below
-            if (y != y) ymax = __builtin_inff();
+        for (int e0 = lane; e0 < npix; e0 += 64 * kPass1Loads) {
+            float yv[kPass1Loads];
+#pragma unroll
+            for (int u = 0; u < kPass1Loads; u++) {
+                const int e = min(e0 + 64 * u, npix - 1);
+                const int jj = udiv(e, dk);
+                yv[u] = to_f32(sino[sino_off(sc, p, j0 + jj, k0 + e - jj * nk)]);
+            }
+#pragma unroll
+            for (int u = 0; u < kPass1Loads; u++) {
+                ymax = fmaxf(ymax, fabsf(yv[u]));   // a NaN sample is caught below
+                if (yv[u] != yv[u]) ymax = __builtin_inff();
+            }
         }
     }
     ymax = wave_max(ymax);
