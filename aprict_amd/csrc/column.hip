@@ -725,9 +725,14 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                             wb[q] = c - hp;
                             wa[q] = ce - c;
                             hp = ce;
+                            // z step of this segment; kept opaque so that the
+                            // two addresses and the next offset take one add
+                            // each (not an add and a select per address)
+                            int d = cr ? dzb : 0;
+                            asm volatile("" : "+v"(d));
                             a0[q] = ov[q] + zb;
-                            zb += cr ? dzb : 0;
-                            a1[q] = ov[q] + zb;
+                            a1[q] = a0[q] + d;
+                            zb += d;
                             pz += cr ? gz : 0.f;
                             hz = tcross(pz, Z.r, Z.c);
                         }
