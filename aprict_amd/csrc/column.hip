@@ -709,18 +709,24 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                     asm volatile("" : "+v"(h4.x), "+v"(h4.y), "+v"(h4.z), "+v"(h4.w), "+v"(o4.x), "+v"(o4.y),
                                  "+v"(o4.z), "+v"(o4.w));
                     // one chunk of 4 segments: weights and addresses first, then
-                    // the adds, so no LDS read is outstanding while they issue;
-                    // all 8 adds issue (zero terms add nothing), a fixed count of
-                    // LDS operations after the read ahead
+                    // the adds, so no LDS read is outstanding while they issue.
+                    // A lane adds the part after a z crossing only where it
+                    // crossed (about one segment in 1/|slope|): the LDS
+                    // atomics, not the arithmetic, bound the walk, and the
+                    // masked add costs less.  The part before is added
+                    // unmasked; it is rarely empty, and a mask there costs
+                    // more in branches and waits than it saves
                     auto chunk = [&](const float4& hc4, const int4& oc4) {
                         const float hv[4] = {hc4.x, hc4.y, hc4.z, hc4.w};
                         const int ov[4] = {oc4.x, oc4.y, oc4.z, oc4.w};
                         float wb[4], wa[4];
                         int a0[4], a1[4];
+                        bool crs[4];
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
                             const float ce = __builtin_amdgcn_fmed3f(hv[q], t0, t1);
                             const bool cr = hz < ce;
+                            crs[q] = cr;
                             const float c = cr ? hz : ce;
                             wb[q] = c - hp;
                             wa[q] = ce - c;
@@ -743,7 +749,8 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
                             atomicAdd(reinterpret_cast<unsigned long long*>(accb + a0[q]), fx_term(wb[q], R));
-                            atomicAdd(reinterpret_cast<unsigned long long*>(accb + a1[q]), fx_term(wa[q], R));
+                            if (crs[q])
+                                atomicAdd(reinterpret_cast<unsigned long long*>(accb + a1[q]), fx_term(wa[q], R));
                         }
                     };
                     // two chunks per turn with alternating list registers
