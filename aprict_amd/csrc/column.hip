@@ -144,9 +144,11 @@ __device__ __forceinline__ float walk_rows_general(const Tv* __restrict__ vol, c
 }
 
 // The same walk when no segment holds more than one z crossing of any row
-// of the wave: straight-line code over 8 segments at a time, so that 16
-// gathers are in flight per lane; a segment without a crossing adds a zero
-// weight.  The segment list is read with wave-uniform addresses (broadcast).
+// of the wave: straight-line code over 8 segments at a time, so that up to
+// 16 gathers are in flight per lane.  The voxel after a z crossing is
+// gathered only in lanes that crossed (others add a zero weight times 0):
+// the masked gathers relieve the memory pipeline, about 4% of the kernel.
+// The segment list is read with wave-uniform addresses (broadcast).
 template <typename Tv>
 __device__ __forceinline__ float walk_rows_single(const Tv* __restrict__ vol, const float* he, const uint32_t* ob,
                                                   int s0, int s1, float t0, float t1, float hz, float pz,
@@ -173,7 +175,7 @@ __device__ __forceinline__ float walk_rows_single(const Tv* __restrict__ vol, co
             pz += cr ? gz : 0.f;
             hz = tcross(pz, rz, cz);
             vb[q] = vget(vol, a0);
-            va[q] = vget(vol, ov[q] + zoff);
+            va[q] = cr ? vget(vol, ov[q] + zoff) : 0.f;
         }
 #pragma unroll
         for (int q = 0; q < 8; q++) {
