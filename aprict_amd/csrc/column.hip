@@ -426,13 +426,35 @@ __device__ __forceinline__ void wave_sync_lds() {
 // Detector footprint [j0, j1] x [k0, k1] of the brick [blo, bhi) in view vr
 // (wave-uniform; lanes 0-7 project one corner each).  Returns false when it
 // is empty.
+// Pixel range of the corners' projections [jmn, jmx] x [kmn, kmx], widened
+// by half a pixel and clipped to the detector; the whole detector when a
+// corner does not project (bad).
+__device__ __forceinline__ bool footprint_range(float jmn, float jmx, float kmn, float kmx, bool bad, uint32_t nu,
+                                                uint32_t nv, int& j0, int& j1, int& k0, int& k1) {
+    if (bad) {
+        j0 = 0; j1 = (int)nu - 1; k0 = 0; k1 = (int)nv - 1;
+    } else {
+        j0 = max(0, (int)floorf(jmn - 0.501f));
+        j1 = min((int)nu - 1, (int)ceilf(jmx + 0.501f));
+        k0 = max(0, (int)floorf(kmn - 0.501f));
+        k1 = min((int)nv - 1, (int)ceilf(kmx + 0.501f));
+    }
+    return j0 <= j1 && k0 <= k1;
+}
+
+__device__ __forceinline__ void brick_corner(const int blo[3], const int bhi[3], const float cen[3], int c,
+                                             float X[3]) {
+    X[0] = (float)((c & 4) ? bhi[0] : blo[0]) - cen[0];
+    X[1] = (float)((c & 2) ? bhi[1] : blo[1]) - cen[1];
+    X[2] = (float)((c & 1) ? bhi[2] : blo[2]) - cen[2];
+}
+
 __device__ inline bool brick_footprint(const ViewRays& vr, const int blo[3], const int bhi[3], const float cen[3],
                                        uint32_t nu, uint32_t nv, int lane, int& j0, int& j1, int& k0, int& k1) {
     float jmn = AP_BIG, jmx = -AP_BIG, kmn = AP_BIG, kmx = -AP_BIG, bad = 0.f;
     if (lane < 8) {
-        const float X[3] = {(float)((lane & 4) ? bhi[0] : blo[0]) - cen[0],
-                            (float)((lane & 2) ? bhi[1] : blo[1]) - cen[1],
-                            (float)((lane & 1) ? bhi[2] : blo[2]) - cen[2]};
+        float X[3];
+        brick_corner(blo, bhi, cen, lane, X);
         float ju, kv;
         if (proj_pt(vr, X, ju, kv)) {
             jmn = jmx = ju;
@@ -443,15 +465,27 @@ __device__ inline bool brick_footprint(const ViewRays& vr, const int blo[3], con
     }
     jmn = lanes8_min(jmn); jmx = lanes8_max(jmx);
     kmn = lanes8_min(kmn); kmx = lanes8_max(kmx);
-    if (lanes8_max(bad) > 0.f) {
-        j0 = 0; j1 = (int)nu - 1; k0 = 0; k1 = (int)nv - 1;
-    } else {
-        j0 = max(0, (int)floorf(jmn - 0.501f));
-        j1 = min((int)nu - 1, (int)ceilf(jmx + 0.501f));
-        k0 = max(0, (int)floorf(kmn - 0.501f));
-        k1 = min((int)nv - 1, (int)ceilf(kmx + 0.501f));
+    return footprint_range(jmn, jmx, kmn, kmx, lanes8_max(bad) > 0.f, nu, nv, j0, j1, k0, k1);
+}
+
+// The same footprint found by one lane alone, its 8 corners in turn (for one
+// view per lane): the same projections and extremes, hence the same range.
+__device__ inline bool lane_footprint(const ViewRays& vr, const int blo[3], const int bhi[3], const float cen[3],
+                                      uint32_t nu, uint32_t nv, int& j0, int& j1, int& k0, int& k1) {
+    float jmn = AP_BIG, jmx = -AP_BIG, kmn = AP_BIG, kmx = -AP_BIG;
+    bool bad = false;
+    for (int c = 0; c < 8; c++) {
+        float X[3];
+        brick_corner(blo, bhi, cen, c, X);
+        float ju, kv;
+        if (proj_pt(vr, X, ju, kv)) {
+            jmn = fminf(jmn, ju); jmx = fmaxf(jmx, ju);
+            kmn = fminf(kmn, kv); kmx = fmaxf(kmx, kv);
+        } else {
+            bad = true;
+        }
     }
-    return j0 <= j1 && k0 <= k1;
+    return footprint_range(jmn, jmx, kmn, kmx, bad, nu, nv, j0, j1, k0, k1);
 }
 
 // One fixed-point term: w * wys truncated to an integer (|w * wys| < 2^30;
@@ -498,63 +532,73 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
     WaveArea& W = wa_all[wave];
     const float cen[3] = {0.5f * (float)(rc.lo[0] + rc.hi[0]), 0.5f * (float)(rc.lo[1] + rc.hi[1]),
                           0.5f * (float)(rc.lo[2] + rc.hi[2])};
-    // pass 1: largest |y| over the brick's footprints sets the fixed-point scale
+    // pass 1: largest |y| over the brick's footprints sets the fixed-point
+    // scale.  Wave w takes views w, w + 16, ...: its lanes first find the
+    // footprints of 64 of them at once (lane l: view pb + 16 l), then the
+    // wave reads each footprint's sinogram values together
     float ymax = 0.f;
-    for (uint32_t p = wave; p < sc.V; p += kAdjWaves) {
-        if (!view_is_upright(specs, p)) continue;   // wave-uniform
-        int j0, j1, k0, k1;
-        if (!brick_footprint(make_view_rays(rc, specs, p), blo, bhi, cen, sc.nu, sc.nv, lane, j0, j1, k0, k1))
-            continue;
-        if (sc.vec16) {
-            // rows [k0, k1] widened to aligned 16-byte groups of kVec values
-            // (nv % kVec == 0 keeps them inside the detector column); the
-            // extra rows only loosen the bound
-            constexpr int kVec = 16 / (int)sizeof(Tp);
-            const int ka = k0 & ~(kVec - 1), ng = ((k1 | (kVec - 1)) - ka + 1) / kVec;
-            const int ngrp = (j1 - j0 + 1) * ng;
-            const UDiv dg = make_udiv(ng);
-            // kPass1Loads loads per lane in flight before any is reduced;
-            // indices past the footprint are clamped to its last group (a
-            // repeated group leaves the maximum unchanged)
-            for (int e0 = lane; e0 < ngrp; e0 += 64 * kPass1Loads) {
-                uint4 raw[kPass1Loads];
-#pragma unroll
-                for (int u = 0; u < kPass1Loads; u++) {
-                    const int e = min(e0 + 64 * u, ngrp - 1);
-                    const int jj = udiv(e, dg);
-                    raw[u] = *reinterpret_cast<const uint4*>(
-                        sino + sino_off(sc, p, j0 + jj, ka + kVec * (e - jj * ng)));
-                }
-#pragma unroll
-                for (int u = 0; u < kPass1Loads; u++) {
-                    const Tp* v = reinterpret_cast<const Tp*>(&raw[u]);
-                    float m = 0.f, sum = 0.f;
-#pragma unroll
-                    for (int q = 0; q < kVec; q++) {
-                        const float y = to_f32(v[q]);
-                        m = fmaxf(m, fabsf(y));
-                        sum += y;
+    for (uint32_t pb = wave; pb < sc.V; pb += 64u * kAdjWaves) {
+        const uint32_t pl = pb + (uint32_t)lane * kAdjWaves;
+        int fj0 = 0, fj1 = -1, fk0 = 0, fk1 = -1;   // empty: not upright or beyond the last view
+        if (pl < sc.V && view_is_upright(specs, pl))
+            lane_footprint(make_view_rays(rc, specs, pl), blo, bhi, cen, sc.nu, sc.nv, fj0, fj1, fk0, fk1);
+        const int nq = (int)min(64u, (sc.V - pb + kAdjWaves - 1) / kAdjWaves);
+        for (int iq = 0; iq < nq; iq++) {
+            const uint32_t p = pb + (uint32_t)iq * kAdjWaves;
+            const int j0 = __builtin_amdgcn_readlane(fj0, iq), j1 = __builtin_amdgcn_readlane(fj1, iq);
+            const int k0 = __builtin_amdgcn_readlane(fk0, iq), k1 = __builtin_amdgcn_readlane(fk1, iq);
+            if (j0 > j1 || k0 > k1) continue;   // wave-uniform
+            if (sc.vec16) {
+                // rows [k0, k1] widened to aligned 16-byte groups of kVec values
+                // (nv % kVec == 0 keeps them inside the detector column); the
+                // extra rows only loosen the bound
+                constexpr int kVec = 16 / (int)sizeof(Tp);
+                const int ka = k0 & ~(kVec - 1), ng = ((k1 | (kVec - 1)) - ka + 1) / kVec;
+                const int ngrp = (j1 - j0 + 1) * ng;
+                const UDiv dg = make_udiv(ng);
+                // kPass1Loads loads per lane in flight before any is reduced;
+                // indices past the footprint are clamped to its last group (a
+                // repeated group leaves the maximum unchanged)
+                for (int e0 = lane; e0 < ngrp; e0 += 64 * kPass1Loads) {
+                    uint4 raw[kPass1Loads];
+    #pragma unroll
+                    for (int u = 0; u < kPass1Loads; u++) {
+                        const int e = min(e0 + 64 * u, ngrp - 1);
+                        const int jj = udiv(e, dg);
+                        raw[u] = *reinterpret_cast<const uint4*>(
+                            sino + sino_off(sc, p, j0 + jj, ka + kVec * (e - jj * ng)));
                     }
-                    ymax = fmaxf(ymax, m);
-                    if (sum != sum) ymax = __builtin_inff();   // NaN (or opposite infinities)
+    #pragma unroll
+                    for (int u = 0; u < kPass1Loads; u++) {
+                        const Tp* v = reinterpret_cast<const Tp*>(&raw[u]);
+                        float m = 0.f, sum = 0.f;
+    #pragma unroll
+                        for (int q = 0; q < kVec; q++) {
+                            const float y = to_f32(v[q]);
+                            m = fmaxf(m, fabsf(y));
+                            sum += y;
+                        }
+                        ymax = fmaxf(ymax, m);
+                        if (sum != sum) ymax = __builtin_inff();   // NaN (or opposite infinities)
+                    }
                 }
+                continue;
             }
-            continue;
-        }
-        const int nk = k1 - k0 + 1, npix = (j1 - j0 + 1) * nk;
-        const UDiv dk = make_udiv(nk);
-        for (int e0 = lane; e0 < npix; e0 += 64 * kPass1Loads) {
-            float yv[kPass1Loads];
-#pragma unroll
-            for (int u = 0; u < kPass1Loads; u++) {
-                const int e = min(e0 + 64 * u, npix - 1);
-                const int jj = udiv(e, dk);
-                yv[u] = to_f32(sino[sino_off(sc, p, j0 + jj, k0 + e - jj * nk)]);
-            }
-#pragma unroll
-            for (int u = 0; u < kPass1Loads; u++) {
-                ymax = fmaxf(ymax, fabsf(yv[u]));   // a NaN sample is caught below
-                if (yv[u] != yv[u]) ymax = __builtin_inff();
+            const int nk = k1 - k0 + 1, npix = (j1 - j0 + 1) * nk;
+            const UDiv dk = make_udiv(nk);
+            for (int e0 = lane; e0 < npix; e0 += 64 * kPass1Loads) {
+                float yv[kPass1Loads];
+    #pragma unroll
+                for (int u = 0; u < kPass1Loads; u++) {
+                    const int e = min(e0 + 64 * u, npix - 1);
+                    const int jj = udiv(e, dk);
+                    yv[u] = to_f32(sino[sino_off(sc, p, j0 + jj, k0 + e - jj * nk)]);
+                }
+    #pragma unroll
+                for (int u = 0; u < kPass1Loads; u++) {
+                    ymax = fmaxf(ymax, fabsf(yv[u]));   // a NaN sample is caught below
+                    if (yv[u] != yv[u]) ymax = __builtin_inff();
+                }
             }
         }
     }
