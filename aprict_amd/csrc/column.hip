@@ -51,6 +51,12 @@ __device__ __forceinline__ float sub_off(uint32_t s, uint32_t ss) {
 }
 
 // Volume element at offset o: bytes for 32-bit offsets, elements for 64-bit.
+// Volume element at 32-bit byte offset o, unconverted.
+template <typename Tv>
+__device__ __forceinline__ Tv vraw(const Tv* __restrict__ v, uint32_t o) {
+    return *reinterpret_cast<const Tv*>(reinterpret_cast<const char*>(v) + o);
+}
+
 template <typename Tv, typename OffT>
 __device__ __forceinline__ float vget(const Tv* __restrict__ v, OffT o) {
     if constexpr (sizeof(OffT) == 4)
@@ -145,9 +151,10 @@ __device__ __forceinline__ float walk_rows_general(const Tv* __restrict__ vol, c
 
 // The same walk when no segment holds more than one z crossing of any row
 // of the wave: straight-line code over 8 segments at a time, so that up to
-// 16 gathers are in flight per lane.  The voxel after a z crossing is
-// gathered only in lanes that crossed (others add a zero weight times 0):
-// the masked gathers relieve the memory pipeline, about 4% of the kernel.
+// 16 gathers are in flight per lane.  For 32-bit volumes the voxel after a
+// z crossing is gathered only in lanes that crossed (others add a zero
+// weight times 0): the masked gathers relieve the memory pipeline, about 4%
+// of the kernel.
 // The segment list is read with wave-uniform addresses (broadcast).
 template <typename Tv>
 __device__ __forceinline__ float walk_rows_single(const Tv* __restrict__ vol, const float* he, const uint32_t* ob,
@@ -161,7 +168,8 @@ __device__ __forceinline__ float walk_rows_single(const Tv* __restrict__ vol, co
         const uint4 o1 = *reinterpret_cast<const uint4*>(ob + s + 4);
         const float hv[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
         const uint32_t ov[8] = {o0.x, o0.y, o0.z, o0.w, o1.x, o1.y, o1.z, o1.w};
-        float wb[8], wa[8], vb[8], va[8];
+        float wb[8], wa[8];
+        Tv vb[8], va[8];   // raw values, converted in the accumulation below
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const float ce = __builtin_amdgcn_fmed3f(hv[q], t0, t1);
@@ -174,13 +182,20 @@ __device__ __forceinline__ float walk_rows_single(const Tv* __restrict__ vol, co
             zoff += cr ? dz : 0u;
             pz += cr ? gz : 0.f;
             hz = tcross(pz, rz, cz);
-            vb[q] = vget(vol, a0);
-            va[q] = cr ? vget(vol, ov[q] + zoff) : 0.f;
+            vb[q] = vraw(vol, a0);
+            // 32-bit volumes: the gather after a crossing only where there is
+            // one (Tv{} is 0.0).  16-bit volumes gather it always: the compiler
+            // moves their conversion into the branch, whose wait for the load
+            // would then stall every gather in flight
+            if constexpr (sizeof(Tv) == 4)
+                va[q] = cr ? vraw(vol, ov[q] + zoff) : Tv{};
+            else
+                va[q] = vraw(vol, ov[q] + zoff);
         }
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            acc = __builtin_fmaf(wb[q], vb[q], acc);
-            acc = __builtin_fmaf(wa[q], va[q], acc);
+            acc = __builtin_fmaf(wb[q], to_f32(vb[q]), acc);
+            acc = __builtin_fmaf(wa[q], to_f32(va[q]), acc);
         }
     }
     return acc;
