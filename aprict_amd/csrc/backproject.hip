@@ -284,6 +284,13 @@ __device__ __forceinline__ float sample_det(const BpCfg& c, const ViewMap& m, co
                 const TapPair<Tp> pa = *reinterpret_cast<const TapPair<Tp>*>(img + (ja * (int32_t)c.s1 + kc));
                 const TapPair<Tp> pb = *reinterpret_cast<const TapPair<Tp>*>(img + (jb * (int32_t)c.s1 + kc));
                 const bool klo = k0 < 0, khi = k0 + 1 >= (int)c.nv;
+                if (__ballot(klo || khi || j0 < 0 || j0 + 1 >= (int)c.nu) == 0) {
+                    // no tap of the wave's samples leaves the detector: the same
+                    // blend without the edge selects
+                    const float t1 = to_f32(pa.lo) * (1.f - a) + to_f32(pb.lo) * a;
+                    const float t2 = to_f32(pa.hi) * (1.f - a) + to_f32(pb.hi) * a;
+                    return t1 * (1.f - b) + t2 * b;
+                }
                 float y00 = klo ? 0.f : (khi ? to_f32(pa.hi) : to_f32(pa.lo));
                 float y01 = khi ? 0.f : (klo ? to_f32(pa.lo) : to_f32(pa.hi));
                 float y10 = klo ? 0.f : (khi ? to_f32(pb.hi) : to_f32(pb.lo));
