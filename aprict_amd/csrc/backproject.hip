@@ -580,7 +580,8 @@ __device__ __forceinline__ float gbc_weight(const GbcSoft& g, const float S[3], 
     const float cr = fminf(1.f, fabsf(r0 * S[0] + r1 * S[1]) * il * rsqrtf(rxy2));
     const float dr = cr - rxy * g.inv_R;
     const float ce = rxy * ir;
-    const float w = ce * ce * ce * cr / fmaxf(1e-9f, cr * cr + dr * dr);
+    // hardware reciprocal (about 1 ulp) instead of an IEEE division
+    const float w = ce * ce * ce * cr * __builtin_amdgcn_rcpf(fmaxf(1e-9f, cr * cr + dr * dr));
     return w * (soft_win(g.v, fabsf(r2) * ir) * soft_win(g.h, sqrtf(fmaxf(0.f, 1.f - cr * cr))));
 }
 
