@@ -101,3 +101,24 @@ def test_column_adjoint_propagates_nan(device):
     hit = np.isnan(want)
     assert hit.any() and np.isnan(got[hit]).all()
     assert np.isfinite(got).any()   # bricks whose footprints miss the pixel stay finite
+
+
+def test_forward_propagates_infinite_voxel(device):
+    # a +inf voxel: rays through it sum to +inf (as the oracle's sum of w * v
+    # does), the others stay finite and match the oracle
+    rng = np.random.default_rng(43)
+    shape, w = (20, 18, 22), 0.6
+    pg = ph.circular_geometry(8, 30, 28, 1.0, R=30.0, L=60.0)
+    x = rng.uniform(0.2, 1.0, shape).astype(np.float32)
+    x[10, 9, 11] = np.inf
+    cfg = cbfp.ConfigProject()
+    ps = cbfp.ProjectionSet(pg)
+    cbfp.project(cfg, Volume(x, voxwid=w), ps, block=True)
+    got = ps.data.cpu().numpy()
+    want = orc.project_voxel(cfg.contents, pg.to_proj_spec_array(), chunk_for(shape, w), x,
+                             np.zeros((8, 30, 28)))
+    hit = np.isinf(want)
+    assert hit.any() and not np.isnan(got).any()
+    np.testing.assert_array_equal(np.isinf(got), hit)
+    assert (got[hit] > 0).all()
+    assert rel_l2(got[~hit], want[~hit]) <= 1e-5
