@@ -151,10 +151,9 @@ __device__ __forceinline__ float walk_rows_general(const Tv* __restrict__ vol, c
 
 // The same walk when no segment holds more than one z crossing of any row
 // of the wave: straight-line code over 8 segments at a time, so that up to
-// 16 gathers are in flight per lane.  For 32-bit volumes the voxel after a
-// z crossing is gathered only in lanes that crossed (others add a zero
-// weight times 0): the masked gathers relieve the memory pipeline, about 4%
-// of the kernel.
+// 16 gathers are in flight per lane.  The voxel after a z crossing is
+// gathered only in lanes that crossed (others add a zero weight times 0):
+// the masked gathers relieve the memory pipeline, about 4% of the kernel.
 // The segment list is read with wave-uniform addresses (broadcast).
 template <typename Tv>
 __device__ __forceinline__ float walk_rows_single(const Tv* __restrict__ vol, const float* he, const uint32_t* ob,
@@ -183,14 +182,13 @@ __device__ __forceinline__ float walk_rows_single(const Tv* __restrict__ vol, co
             pz += cr ? gz : 0.f;
             hz = tcross(pz, rz, cz);
             vb[q] = vraw(vol, a0);
-            // 32-bit volumes: the gather after a crossing only where there is
-            // one (Tv{} is 0.0).  16-bit volumes gather it always: the compiler
-            // moves their conversion into the branch, whose wait for the load
-            // would then stall every gather in flight
-            if constexpr (sizeof(Tv) == 4)
-                va[q] = cr ? vraw(vol, ov[q] + zoff) : Tv{};
-            else
-                va[q] = vraw(vol, ov[q] + zoff);
+            // the gather after a crossing only where there is one (Tv{} is
+            // 0.0).  The zero is opaque: otherwise the compiler folds the
+            // conversion of 16-bit values into the branch, whose wait for the
+            // load then stalls every gather in flight
+            Tv zero = Tv{};
+            asm volatile("" : "+v"(zero));
+            va[q] = cr ? vraw(vol, ov[q] + zoff) : zero;
         }
 #pragma unroll
         for (int q = 0; q < 8; q++) {
