@@ -150,7 +150,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ap_last_error": (ctypes.c_char_p, []),
     "ap_version": (_I32, []),
     "ap_struct_size": (_I32, [_I32]),
-    "ap_project_voxel": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _P, _P]),
+    "ap_project_voxel": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _P]),
     "ap_backproject": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _A3, _P, _P]),
     "ap_backproject_matched": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _F32, _P]),
     "ap_project_forbild": (_I32, [_P, _P, _U32, _P, _U32, _A3, _P, _P, _P]),
@@ -168,7 +168,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ap_volume_softmask_cylinder": (_I32, [_A3, _P, _F32, _P]),
     "ap_array_dot": (_I32, [_A3, _P, _A3, _P, _P, _P]),
     "ap_tv_gradient": (_I32, [_A3, _P, _A3, _P, _F32, _F32, _P]),
-    "ap_sirt_iterate": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _P, _P]),
+    "ap_sirt_iterate": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _A3, _P, _P]),
     "ap_tune_count": (_I32, [_I32]),
     "ap_tune_get": (_I32, [_I32]),
     "ap_tune_set": (_I32, [_I32, _I32]),

@@ -52,7 +52,8 @@ class Projector:
             out = torch.zeros(self.sino_shape, dtype=torch.float32, device=x.device)
         _abi.check(_abi.lib().ap_project_voxel(
             _abi.struct_ptr(self._cfg), _abi.ptr(self._specs), self.sino_shape[0], _abi.struct_ptr(self._chunk),
-            _abi.array3(x), _abi.ptr(x), _abi.array3(out), _abi.ptr(out), _abi.ptr(b) if b is not None else None,
+            _abi.array3(x), _abi.ptr(x), _abi.array3(out), _abi.ptr(out),
+            _abi.array3(b) if b is not None else None, _abi.ptr(b) if b is not None else None,
             _abi.stream_handle(x.device)), "A")
         return out
 

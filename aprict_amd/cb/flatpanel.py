@@ -466,7 +466,7 @@ def project(config: ConfigProject, volume: Volume | FBVolume, projs: ProjectionS
         vd = _abi.array3(vdat)
         _abi.check(lib.ap_project_voxel(_abi.struct_ptr(cfg), _abi.ptr(specs), nprojs,
                                         _abi.struct_ptr(chunk), vd, _abi.ptr(vdat), sd, _abi.ptr(data),
-                                        None, stream), "project")
+                                        None, None, stream), "project")
     elif type(volume) is FBVolume:
         pruned = volume.prune_to_bbox(*projs.get_bbox())
         shapes = pruned.shapes_device(data.device)

@@ -1,19 +1,20 @@
 // column.hip -- column-coherent Siddon forward projector and matched adjoint
 // for views whose detector v axis is parallel to z (see column.h).
 //
-// Forward: a workgroup takes one detector (sub-)column of one view and 256
-// consecutive rows.  It merges the column's x- and y-plane crossings into a
-// segment list in LDS (each crossing computes its own rank, so the list is
-// built by all threads at once), then every lane walks the list, inserting
-// its own z crossings: per voxel a lane does one LDS broadcast read, a few
-// FP32 operations and one gather.  Row-adjacent lanes read z-adjacent voxels
-// of the same column, so each gather touches one or two cache lines.
+// Forward: a workgroup takes one detector (sub-)column of one view and 128
+// consecutive rows (64 or 256 in the other launch-shape variants).  It merges
+// the column's x- and y-plane crossings into a segment list in LDS (each
+// crossing computes its own rank, so the list is built by all threads at
+// once), then every lane walks the list, inserting its own z crossings: per
+// voxel a lane does one LDS broadcast read, a few FP32 operations and one
+// gather.  Row-adjacent lanes read z-adjacent voxels of the same column, so
+// each gather touches one or two cache lines.
 //
 // Adjoint: a workgroup owns a brick of kTX x kTY x kTZ voxels accumulated in
-// LDS over all views.  Per view, each wave takes units (detector column,
-// 64 rows) of the brick's footprint, builds that column's segment list inside
-// the brick, and every lane scatters weight * y along its ray with LDS float
-// atomics.  Segment boundaries and weights are those of the forward walk.
+// LDS over all views as 64-bit fixed point (ds_add_u64).  Per view, each wave
+// takes up to kWCols (sub-)columns of the brick's footprint, builds their
+// segment lists inside the brick, and every lane scatters weight * y along
+// its ray.  Segment boundaries and weights are those of the forward walk.
 #include "column.h"
 
 namespace {
@@ -27,6 +28,7 @@ constexpr int kColRows[3] = {128, 256, 64};
 struct SinoCfgC {
     uint32_t V, nu, nv;
     int64_t s0, s1, s2;
+    int64_t b0, b1, b2;   // element strides of the fused residual's b (forward)
     uint32_t vec16;    // f32 or f16, v contiguous, rows 16-byte aligned, nv a multiple of 16 bytes
     float unit;        // adjoint fixed-point unit 2^b (adjoint_fixed_point_unit)
     uint32_t o32;      // every element offset fits 31 bits: 32-bit address arithmetic
@@ -352,7 +354,8 @@ __global__ __launch_bounds__(kRowsF) void k_project_columns(
     if (ssu > 1) tot /= (float)ssu;
     tot *= (float)(1.0 / rc.inv_vw);
     const int64_t o = (int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2;
-    const float base = to_f32(b ? b[o] : sino[o]);
+    const float base =
+        to_f32(b ? b[(int64_t)p * sc.b0 + (int64_t)j * sc.b1 + (int64_t)k * sc.b2] : sino[o]);
     sino[o] = from_f32<Tp>(__builtin_fmaf(mba, tot, base));
 }
 
@@ -501,29 +504,30 @@ __device__ inline bool lane_footprint(const ViewRays& vr, const int blo[3], cons
     return footprint_range(jmn, jmx, kmn, kmx, bad, nu, nv, j0, j1, k0, k1);
 }
 
-// One fixed-point term: w * wys truncated to an integer (|w * wys| < 2^30;
-// the truncation bias is below one unit, about 1e-9 of the largest term).
+// floor(x + 0.5) as an integer in one instruction (round half up: the
+// quantisation error of a term is uniform in (-1/2, 1/2] of a unit, no bias).
+__device__ __forceinline__ int cvt_rpi(float x) {
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// One fixed-point term: w * wys rounded to the nearest integer (|w * wys| <
+// 2^30), sign-extended to 64 bits.
 __device__ __forceinline__ unsigned long long fx_term(float w, float wys) {
-    return (unsigned long long)(long long)(int)(w * wys);
+    return (unsigned long long)(long long)cvt_rpi(w * wys);
 }
 
-// All terms of one ray share the sign of wys, so the high word of the 64-bit
-// two's-complement term is a per-ray constant: 0 when wys >= 0, all ones
-// otherwise.  A ray with wys < 0 adds trunc(w * wys - 1) <= -1, which keeps
-// the constant high word exact (zero weights included); the extra unit is
-// within the quantisation (about 2e-9 of the largest term).
-struct FxRay {
-    float wys, bias;
-    uint32_t hi;
-};
-
-__device__ __forceinline__ FxRay fx_ray(float wys) {
-    return FxRay{wys, wys < 0.f ? -1.f : 0.f, wys < 0.f ? 0xFFFFFFFFu : 0u};
-}
-
-__device__ __forceinline__ unsigned long long fx_term(float w, const FxRay& R) {
-    const uint32_t lo = (uint32_t)(int)__builtin_fmaf(w, R.wys, R.bias);
-    return ((unsigned long long)R.hi << 32) | lo;
+// A term of the walk.  In a brick where every w * y * scale is >= 0 (the
+// footprint's sinogram values and the scale share one sign; kSigned false)
+// every rounded term is >= 0, so its 64-bit high word is the constant 0.
+// Otherwise (kSigned) the high word is the low word's sign, one more
+// instruction per term.
+template <bool kSigned>
+__device__ __forceinline__ unsigned long long fx_term_walk(float w, float wys) {
+    const int lo = cvt_rpi(w * wys);
+    const uint32_t hi = kSigned ? (uint32_t)(lo >> 31) : 0u;
+    return ((unsigned long long)hi << 32) | (uint32_t)lo;
 }
 
 template <typename Tv, typename Tp, typename Tm, typename IdxT>
@@ -534,7 +538,7 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
     uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t per_xcd) {
     __shared__ unsigned long long acc[kTX * kTY * kTZ];
     __shared__ __align__(16) WaveArea wa_all[kAdjWaves];
-    __shared__ float wmax[kAdjWaves];
+    __shared__ float wmin[kAdjWaves], wmax[kAdjWaves];
     const uint32_t L = blockIdx.x;
     const uint32_t logical = (L % kXcdC) * per_xcd + L / kXcdC;
     if (logical >= nbx * nby * nbz) return;
@@ -545,11 +549,12 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
     WaveArea& W = wa_all[wave];
     const float cen[3] = {0.5f * (float)(rc.lo[0] + rc.hi[0]), 0.5f * (float)(rc.lo[1] + rc.hi[1]),
                           0.5f * (float)(rc.lo[2] + rc.hi[2])};
-    // pass 1: largest |y| over the brick's footprints sets the fixed-point
-    // scale.  Wave w takes views w, w + 16, ...: its lanes first find the
-    // footprints of 64 of them at once (lane l: view pb + 16 l), then the
-    // wave reads each footprint's sinogram values together
-    float ymax = 0.f;
+    // pass 1: the range [ylo, yhi] of the sinogram values over the brick's
+    // footprints sets the fixed-point scale (largest |y|) and tells whether
+    // every term has one sign.  Wave w takes views w, w + 16, ...: its lanes
+    // first find the footprints of 64 of them at once (lane l: view
+    // pb + 16 l), then the wave reads each footprint's values together
+    float ylo = 0.f, yhi = 0.f;
     for (uint32_t pb = wave; pb < sc.V; pb += 64u * kAdjWaves) {
         const uint32_t pl = pb + (uint32_t)lane * kAdjWaves;
         int fj0 = 0, fj1 = -1, fk0 = 0, fk1 = -1;   // empty: not upright or beyond the last view
@@ -584,15 +589,17 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
     #pragma unroll
                     for (int u = 0; u < kPass1Loads; u++) {
                         const Tp* v = reinterpret_cast<const Tp*>(&raw[u]);
-                        float m = 0.f, sum = 0.f;
+                        float mn = 0.f, mx = 0.f, sum = 0.f;
     #pragma unroll
                         for (int q = 0; q < kVec; q++) {
                             const float y = to_f32(v[q]);
-                            m = fmaxf(m, fabsf(y));
+                            mn = fminf(mn, y);
+                            mx = fmaxf(mx, y);
                             sum += y;
                         }
-                        ymax = fmaxf(ymax, m);
-                        if (sum != sum) ymax = __builtin_inff();   // NaN (or opposite infinities)
+                        ylo = fminf(ylo, mn);
+                        yhi = fmaxf(yhi, mx);
+                        if (sum != sum) yhi = __builtin_inff();   // NaN (or opposite infinities)
                     }
                 }
                 continue;
@@ -609,19 +616,31 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                 }
     #pragma unroll
                 for (int u = 0; u < kPass1Loads; u++) {
-                    ymax = fmaxf(ymax, fabsf(yv[u]));   // a NaN sample is caught below
-                    if (yv[u] != yv[u]) ymax = __builtin_inff();
+                    ylo = fminf(ylo, yv[u]);   // a NaN sample is caught below
+                    yhi = fmaxf(yhi, yv[u]);
+                    if (yv[u] != yv[u]) yhi = __builtin_inff();
                 }
             }
         }
     }
-    ymax = wave_max(ymax);
-    if (lane == 0) wmax[wave] = ymax;
+    ylo = wave_min(ylo);
+    yhi = wave_max(yhi);
+    if (lane == 0) {
+        wmin[wave] = ylo;
+        wmax[wave] = yhi;
+    }
     for (int e = tid; e < kTX * kTY * kTZ; e += kAdjT) acc[e] = 0ull;
     __syncthreads();
-    ymax = 0.f;
-    for (int w = 0; w < kAdjWaves; w++) ymax = fmaxf(ymax, wmax[w]);
-    const float amax = ymax * fabsf(scale);
+    ylo = 0.f;
+    yhi = 0.f;
+    for (int w = 0; w < kAdjWaves; w++) {
+        ylo = fminf(ylo, wmin[w]);
+        yhi = fmaxf(yhi, wmax[w]);
+    }
+    const float amax = fmaxf(yhi, -ylo) * fabsf(scale);
+    // every w * y * scale >= 0 (or <= 0 with the sign folded into the scale):
+    // the walk's terms need no sign extension
+    const bool one_sign = scale >= 0.f ? ylo >= 0.f : yhi <= 0.f;
     if (amax == 0.f) return;   // block-uniform: nothing to add
     if (!(amax < __builtin_inff())) {
         // non-finite data: the sum is not representable; propagate NaN
@@ -636,6 +655,9 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
     // |w * sec| <= sqrt(3) < 2, so every term is below 2 * unit in magnitude
     const float S = sc.unit / amax;
     const float scaleS = scale * S;
+    // the views, with or without sign extension of the terms (block-uniform)
+    auto all_views = [&](auto sign_kind) {
+    constexpr bool kSigned = decltype(sign_kind)::value;
     for (uint32_t p = wave; p < sc.V; p += kAdjWaves) {
         if (!view_is_upright(specs, p)) continue;   // wave-uniform
         wave_sync_lds();   // the previous view is done with the area
@@ -754,12 +776,11 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                     // with the chunk whose last entry reaches t1.  The walk
                     // starts at segment 0: segments that end before t0 clamp
                     // to [t0, t0], weight 0 and hold no z crossing (the next
-                    // one, hz, lies beyond t0), so they add zero terms (one
-                    // rounding unit for a negative ray) to cells of the brick;
+                    // one, hz, lies beyond t0), so they add zero terms to
+                    // cells of the brick;
                     // most rows enter the brick at h0, where the walk starts
                     // anyway, and a search for the first segment would cost
                     // more than the few extra chunks of the others
-                    const FxRay R = fx_ray(wys);
                     int s = 0;
                     float4 h4 = *reinterpret_cast<const float4*>(he + s);
                     int4 o4 = *reinterpret_cast<const int4*>(ob + s);
@@ -807,9 +828,11 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
                                      "+v"(a1[1]), "+v"(a1[2]), "+v"(a1[3]));
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
-                            atomicAdd(reinterpret_cast<unsigned long long*>(accb + a0[q]), fx_term(wb[q], R));
+                            atomicAdd(reinterpret_cast<unsigned long long*>(accb + a0[q]),
+                                      fx_term_walk<kSigned>(wb[q], wys));
                             if (crs[q])
-                                atomicAdd(reinterpret_cast<unsigned long long*>(accb + a1[q]), fx_term(wa[q], R));
+                                atomicAdd(reinterpret_cast<unsigned long long*>(accb + a1[q]),
+                                          fx_term_walk<kSigned>(wa[q], wys));
                         }
                     };
                     // two chunks per turn with alternating list registers
@@ -857,6 +880,11 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
             }
         }
     }
+    };
+    if (one_sign)
+        all_views(std::false_type{});
+    else
+        all_views(std::true_type{});
     __syncthreads();
     const double inv_s = 1.0 / (double)S;
     for (int e = tid; e < kTX * kTY * kTZ; e += kAdjT) {
@@ -879,7 +907,7 @@ __global__ __launch_bounds__(kAdjT) void k_backproject_columns(
 int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32_t nv, int64_t s0, int64_t s1,
                                int64_t s2, float mba, uint32_t ssu, uint32_t ssv, const ap_proj_spec* specs,
                                const ap_array3* vol_desc, const void* vol, int32_t sino_dtype, void* sino,
-                               const void* b, bool idx64, hipStream_t s) {
+                               const void* b, const int64_t* bs, bool idx64, hipStream_t s) {
     // segment list: at most Nx + Ny + 1 segments plus 8 padding entries
     const uint32_t cap = ((uint32_t)(rc.hi[0] - rc.lo[0]) + (uint32_t)(rc.hi[1] - rc.lo[1]) + 9 + 3) & ~3u;
     // 32-bit byte offsets when the whole volume spans less than 4 GiB
@@ -887,7 +915,7 @@ int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32
     const bool bytes32 = !idx64 && (uint64_t)(array3_max_offset(vol_desc) + 1) * esz <= 0xFFFFFFFFull;
     const size_t shm = ((cap * sizeof(float) + 15) & ~size_t(15)) + cap * (bytes32 ? 4 : 8);
     if (shm > 60 * 1024) return AP_ERR_UNSUPPORTED;
-    SinoCfgC sc{V, nu, nv, s0, s1, s2, 0u, 0.f};
+    SinoCfgC sc{V, nu, nv, s0, s1, s2, bs[0], bs[1], bs[2], 0u, 0.f};
     const int var = ap_tune_variant(AP_TUNE_PROJECT_COLUMN);
     const uint32_t rows = (uint32_t)kColRows[var];
     ColMap cm;
@@ -938,7 +966,7 @@ int32_t launch_backproject_columns(const RayCfg& rc, uint32_t V, uint32_t nu, ui
                                    const ap_proj_spec* specs, const ap_array3* vol_desc, void* vol,
                                    int32_t mdt, const int64_t* ms, const void* mult, int32_t sino_dtype,
                                    const void* sino, bool idx64, hipStream_t s) {
-    SinoCfgC sc{V, nu, nv, s0, s1, s2, 0u};
+    SinoCfgC sc{V, nu, nv, s0, s1, s2, s0, s1, s2, 0u};
     sc.o32 = (double)(V ? V - 1 : 0) * (double)s0 + (double)(nu ? nu - 1 : 0) * (double)s1 +
                      (double)(nv ? nv - 1 : 0) * (double)s2 < 2147483647.0;
     {

@@ -34,6 +34,7 @@ constexpr int kXcd = 8;      // MI355X: 8 XCDs, workgroups dispatched round-robi
 struct SinoCfg {
     uint32_t V, nu, nv;
     int64_t s0, s1, s2;      // element strides
+    int64_t b0, b1, b2;      // element strides of the fused residual's b
 };
 
 struct TileMap {
@@ -106,7 +107,8 @@ __global__ __launch_bounds__(kWaveV* kRowsU) void k_project_voxel(
     if (ssu > 1) tot /= (float)ssu;
     tot *= (float)(1.0 / rc.inv_vw);   // voxel-length units -> physical
     const int64_t o = (int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2;
-    const float base = to_f32(b ? b[o] : sino[o]);
+    const float base =
+        to_f32(b ? b[(int64_t)p * sc.b0 + (int64_t)j * sc.b1 + (int64_t)k * sc.b2] : sino[o]);
     sino[o] = from_f32<Tp>(__builtin_fmaf(mba, tot, base));
 }
 
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(kAdjThreads, 4) void k_backproject_matched(
 int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32_t nv, int64_t s0, int64_t s1,
                                int64_t s2, float mba, uint32_t ssu, uint32_t ssv, const ap_proj_spec* specs,
                                const ap_array3* vol_desc, const void* vol, int32_t sino_dtype, void* sino,
-                               const void* b, bool idx64, hipStream_t s);
+                               const void* b, const int64_t* bs, bool idx64, hipStream_t s);
 int32_t launch_backproject_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32_t nv, int64_t s0,
                                    int64_t s1, int64_t s2, float scale, uint32_t ssu, uint32_t ssv,
                                    const ap_proj_spec* specs, const ap_array3* vol_desc, void* vol,
@@ -329,9 +331,9 @@ int32_t check_sino(const ap_array3* sd, const void* sino, uint32_t num_projs, Si
     sc.V = sd->shape[0];
     sc.nu = sd->shape[1];
     sc.nv = sd->shape[2];
-    sc.s0 = sd->strides[0];
-    sc.s1 = sd->strides[1];
-    sc.s2 = sd->strides[2];
+    sc.s0 = sc.b0 = sd->strides[0];
+    sc.s1 = sc.b1 = sd->strides[1];
+    sc.s2 = sc.b2 = sd->strides[2];
     return AP_OK;
 }
 
@@ -366,13 +368,28 @@ int32_t make_ray_cfg(const ap_config_cb_project* cfg, const ap_vol_tex_chunk_spe
 extern "C" int32_t ap_project_voxel(const ap_config_cb_project* cfg, const ap_proj_spec* specs,
                                     uint32_t num_projs, const ap_vol_tex_chunk_spec* chunk,
                                     const ap_array3* vol_desc, const void* vol,
-                                    const ap_array3* sino_desc, void* sino, const void* b,
-                                    void* stream) {
+                                    const ap_array3* sino_desc, void* sino,
+                                    const ap_array3* b_desc, const void* b, void* stream) {
     AP_REQUIRE(specs, "specs is NULL");
     int32_t st = ap_check_array3(vol_desc, vol, "volume", true, true, false);
     if (st) return st;
     SinoCfg sc;
     if ((st = check_sino(sino_desc, sino, num_projs, sc, false))) return st;
+    if (b) {
+        // the fused residual's b: same shape and dtype as sino, its own strides
+        AP_REQUIRE(b_desc, "b is given without its descriptor");
+        if ((st = ap_check_array3(b_desc, b, "b", true, false, false))) return st;
+        for (int a = 0; a < 3; a++)
+            AP_REQUIRE(b_desc->shape[a] == sino_desc->shape[a], "b shape != sino shape along axis %d", a);
+        AP_REQUIRE(b_desc->dtype == sino_desc->dtype, "b dtype != sino dtype");
+        AP_REQUIRE(b != sino || (b_desc->strides[0] == sino_desc->strides[0] &&
+                                 b_desc->strides[1] == sino_desc->strides[1] &&
+                                 b_desc->strides[2] == sino_desc->strides[2]),
+                   "b aliases sino with other strides");
+        sc.b0 = b_desc->strides[0];
+        sc.b1 = b_desc->strides[1];
+        sc.b2 = b_desc->strides[2];
+    }
     RayCfg rc;
     if ((st = make_ray_cfg(cfg, chunk, vol_desc, rc))) return st;
     if (num_projs == 0 || sc.nu == 0 || sc.nv == 0) return AP_OK;
@@ -394,8 +411,9 @@ extern "C" int32_t ap_project_voxel(const ap_config_cb_project* cfg, const ap_pr
     const float mba = cfg->multiply_before_adding;
     // upright views go to the column-coherent kernel; the general kernel
     // then only takes the remaining views
+    const int64_t bs[3] = {sc.b0, sc.b1, sc.b2};
     st = launch_project_columns(rc, sc.V, sc.nu, sc.nv, sc.s0, sc.s1, sc.s2, mba, cfg->ssu, cfg->ssv, specs,
-                                vol_desc, vol, sino_desc->dtype, sino, b, idx64, s);
+                                vol_desc, vol, sino_desc->dtype, sino, b, bs, idx64, s);
     if (st != AP_OK && st != AP_ERR_UNSUPPORTED) return st;
     const int skip = st == AP_OK ? 1 : 0;
     return dispatch_dtype(vol_desc->dtype, [&](auto tv) -> int32_t {

@@ -337,15 +337,19 @@ int32_t ap_tv_gradient(const ap_array3* x_desc, const void* x, const ap_array3* 
 int32_t ap_sirt_iterate(const ap_config_cb_backproject* bp_cfg, const ap_proj_spec* specs,
                         uint32_t num_projs, const ap_vol_tex_chunk_spec* chunk,
                         const ap_array3* vol_desc, void* x, const ap_array3* m_desc, const void* M,
-                        const ap_array3* sino_desc, const void* b, void* r, void* stream) {
-    AP_REQUIRE(bp_cfg && b && r, "NULL config, measurements or residual buffer");
+                        const ap_array3* b_desc, const void* b, const ap_array3* r_desc, void* r,
+                        void* stream) {
+    AP_REQUIRE(bp_cfg && b && r && b_desc && r_desc, "NULL config, measurements or residual buffer");
     AP_REQUIRE(b != r, "residual buffer must not alias the measurements");
-    // r = b - A x, written once by the projector epilogue
+    for (int a = 0; a < 3; a++)
+        AP_REQUIRE(r_desc->shape[a] == b_desc->shape[a], "residual shape != measurement shape along axis %d", a);
+    AP_REQUIRE(r_desc->dtype == b_desc->dtype, "residual dtype != measurement dtype");
+    // r = b - A x, written once by the projector epilogue (b and r each with its own strides)
     const ap_config_cb_project fp = {1.f, -1.f, 1u, 1u};
-    int32_t st = ap_project_voxel(&fp, specs, num_projs, chunk, vol_desc, x, sino_desc, r, b, stream);
+    int32_t st = ap_project_voxel(&fp, specs, num_projs, chunk, vol_desc, x, r_desc, r, b_desc, b, stream);
     if (st) return st;
     // x += M (.) BP(r)
-    return ap_backproject(bp_cfg, specs, num_projs, chunk, vol_desc, x, m_desc, M, sino_desc, r, nullptr, nullptr,
+    return ap_backproject(bp_cfg, specs, num_projs, chunk, vol_desc, x, m_desc, M, r_desc, r, nullptr, nullptr,
                           stream);
 }
 

@@ -6,7 +6,7 @@ per-subset step ``accel/lambda`` with ``lambda`` the Rayleigh quotient of
 positive volume; ``iterate()`` does, per subset,
 ``r = z (.) (b - A x)``, ``x += (accel/lambda) B r``.  The residual comes from
 the fused projector epilogue; masks and scaling are single element-wise
-launches; norms are accumulated in FP64.
+library launches; norms are FP64 sums in the library.
 """
 
 from __future__ import annotations
@@ -25,6 +25,29 @@ def _torch():
     import torch  # noqa: PLC0415
 
     return torch
+
+
+def _apply_mask(projs: cbfp.ProjectionSet, zm) -> None:
+    """projs.data *= zm (zm broadcast to the data's shape), one library call."""
+    torch = _torch()
+    data = projs.data
+    m = torch.as_tensor(zm, device=data.device).to(data.dtype).expand(data.shape)
+    flat = projs._flat_data()
+    try:
+        m3 = m.view(flat.shape)
+    except RuntimeError:
+        m3 = m.contiguous().view(flat.shape)
+    _abi.check(_abi.lib().ap_array_mul(_abi.array3(flat), _abi.ptr(flat), _abi.array3(m3), _abi.ptr(m3),
+                                       _abi.stream_handle(data.device)), "mask")
+
+
+def _sq_norm(t) -> float:
+    """sum t^2 accumulated in FP64 on the GPU."""
+    torch = _torch()
+    out = torch.empty(1, dtype=torch.float64, device=t.device)
+    _abi.check(_abi.lib().ap_array_dot(_abi.array3(t), _abi.ptr(t), _abi.array3(t), _abi.ptr(t), _abi.ptr(out),
+                                       _abi.stream_handle(t.device)), "norm")
+    return float(out.item())
 
 
 class Landweber_flatpanel:  # noqa: N801
@@ -71,15 +94,18 @@ class Landweber_flatpanel:  # noqa: N801
             num_iterations=config.rayleigh_quotient_num_iterations)
         for p, zm in zip(self._meas, self._proj_zero_masks):
             if zm is not None:
-                p.data.mul_(zm)
+                _apply_mask(p, zm)
 
     @property
     def rayleigh_quotients(self) -> list[float]:
         return list(self._rayleigh_quotients)
 
     def calculate_rayleigh_quotients(self, num_iterations: int) -> list[float]:
-        """Rayleigh quotient of B A per subset by the power method."""
-        torch = _torch()
+        """Rayleigh quotient of B A per subset by the power method (landweber_vram.py:127-209).
+
+        Per subset: a random positive volume v, then ``num_iterations`` times
+        v <- v / |v|, rq = |A v|^2, v <- B A v.  Norms are FP64 sums on the GPU.
+        """
         use_existing = self._vol.norm == 0
         scratch = self._vol if use_existing else self._vol.zeroed_copy()
         scale = 1.0 / float(np.mean(scratch.shape))
@@ -93,7 +119,7 @@ class Landweber_flatpanel:  # noqa: N801
                 scratch.multiply(1.0 / norm)
                 sp.data.zero_()
                 cbfp.project(self._config_fp, scratch, sp)
-                rq = float(torch.linalg.vector_norm(sp.data.double()).item()) ** 2
+                rq = _sq_norm(sp._flat_data())
                 scratch.fill(0.0)
                 cbfp.backproject(self._config_bp, scratch, sp)
                 norm = scratch.norm
@@ -103,7 +129,7 @@ class Landweber_flatpanel:  # noqa: N801
         return out
 
     def iterate(self) -> None:
-        """One pass over the ordered subsets."""
+        """One pass over the ordered subsets (landweber_vram.py:221-260)."""
         lib = _abi.lib()
         x = self._vol.data
         chunk = self._vol.chunk_spec()
@@ -112,12 +138,13 @@ class Landweber_flatpanel:  # noqa: N801
                                          self._proj_zero_masks):
             b = meas._flat_data()
             r = scratch._flat_data()
+            # r = b - A x in the projector epilogue (b and r with their own strides)
             _abi.check(lib.ap_project_voxel(_abi.struct_ptr(fp), _abi.ptr(meas.specs_device()), b.shape[0],
                                             _abi.struct_ptr(chunk), _abi.array3(x), _abi.ptr(x),
-                                            _abi.array3(r), _abi.ptr(r), _abi.ptr(b),
+                                            _abi.array3(r), _abi.ptr(r), _abi.array3(b), _abi.ptr(b),
                                             _abi.stream_handle(x.device)), "residual")
             if zm is not None:
-                scratch.data.mul_(zm)
+                _apply_mask(scratch, zm)
             _abi.check(lib.ap_array_scale(_abi.array3(r), _abi.ptr(r), self._config.acceleration_factor / rq,
                                           _abi.stream_handle(x.device)), "scale")
             cbfp.backproject(self._config_bp, self._vol, scratch)

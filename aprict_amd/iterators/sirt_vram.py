@@ -32,6 +32,16 @@ def _torch():
     return torch
 
 
+def _mean(t) -> float:
+    """Mean of a 3-D tensor: an FP64 dot product with a broadcast 1 in the library."""
+    torch = _torch()
+    one = torch.ones(1, dtype=t.dtype, device=t.device).expand(t.shape)
+    out = torch.empty(1, dtype=torch.float64, device=t.device)
+    _abi.check(_abi.lib().ap_array_dot(_abi.array3(t), _abi.ptr(t), _abi.array3(one), _abi.ptr(one),
+                                       _abi.ptr(out), _abi.stream_handle(t.device)), "mean")
+    return float(out.item()) / max(t.numel(), 1)
+
+
 class SIRT_flatpanel:  # noqa: N801
     """An iterable SIRT reconstructor."""
 
@@ -62,8 +72,10 @@ class SIRT_flatpanel:  # noqa: N801
         M.fill(scale)
         M.soften_to_xy_cylinder(1.0)
         cbfp.project(self._config_fp, M, p)
-        meanval = float(p.data.double().mean().item())
-        p.data.div_(meanval / scale)
+        pf = p._flat_data()
+        meanval = _mean(pf)
+        _abi.check(_abi.lib().ap_array_scale(_abi.array3(pf), _abi.ptr(pf), scale / meanval,
+                                             _abi.stream_handle(pf.device)), "normalise")
         scale /= meanval / scale
         M.fill(0.0)
         bp_nomult = cbfp.ConfigBackproject(fast_approx=config.fast_backproject, cyl_mask_rad=cyl,
@@ -95,7 +107,7 @@ class SIRT_flatpanel:  # noqa: N801
         _abi.check(_abi.lib().ap_sirt_iterate(
             _abi.struct_ptr(cfg), _abi.ptr(self._meas.specs_device()), b.shape[0], _abi.struct_ptr(chunk),
             _abi.array3(x), _abi.ptr(x), _abi.array3(M), _abi.ptr(M), _abi.array3(b), _abi.ptr(b),
-            _abi.ptr(r), _abi.stream_handle(x.device)), "sirt_iterate")
+            _abi.array3(r), _abi.ptr(r), _abi.stream_handle(x.device)), "sirt_iterate")
 
     @property
     def volume(self) -> Volume:

@@ -234,7 +234,7 @@ def project_streamed(config: cbfp.ConfigProject, volume: HostVolume,
         vd, base = _shifted(dev, off, volume.shape)
         chunk = volume.chunk_spec(off, size)
         _abi.check(lib.ap_project_voxel(_abi.struct_ptr(cfg), _abi.ptr(specs), data.shape[0],
-                                        _abi.struct_ptr(chunk), vd, base, sd, _abi.ptr(data), None,
+                                        _abi.struct_ptr(chunk), vd, base, sd, _abi.ptr(data), None, None,
                                         _abi.stream_handle(data.device)), "project_streamed")
         st.release(i)
     torch.cuda.synchronize(data.device)

@@ -188,17 +188,18 @@ int32_t     ap_struct_size(int32_t which);
  * For every view p, pixel (j,k):
  *   sino[p][j][k] = base + multiply_before_adding *
  *                   mean over ssu x ssv sub-rays of  INT_{source->subpixel} f
- * with base = b[p][j][k] if b != NULL (fused residual, b has sino's shape,
- * dtype and strides) else sino[p][j][k] (accumulate, like the reference's
- * atomicAdd).  f is the piecewise-constant volume chunk `chunk`, stored in
- * `vol` (desc vol_desc: dtype F32/F16/BF16, shape = whole volume).
+ * with base = b[p][j][k] if b != NULL (fused residual) else sino[p][j][k]
+ * (accumulate, like the reference's atomicAdd).  b has sino's shape and
+ * dtype and its own strides (b_desc, required with b; b may be sino itself
+ * with the same strides).  f is the piecewise-constant volume chunk `chunk`,
+ * stored in `vol` (desc vol_desc: dtype F32/F16/BF16, shape = whole volume).
  * Sinogram dtype: F32 or F16.                                              */
 int32_t ap_project_voxel(const ap_config_cb_project* cfg,
                          const ap_proj_spec* specs, uint32_t num_projs,
                          const ap_vol_tex_chunk_spec* chunk,
                          const ap_array3* vol_desc, const void* vol,
                          const ap_array3* sino_desc, void* sino,
-                         const void* b, void* stream);
+                         const ap_array3* b_desc, const void* b, void* stream);
 
 /* ---- backprojection (reference semantics) ------------------------------
  * Replaces the backproject_CB_voltex launches in cb/flatpanel.py:1128-1281
@@ -359,13 +360,15 @@ int32_t ap_tv_gradient(const ap_array3* x_desc, const void* x,
 /* ---- fused SIRT iteration ----------------------------------------------
  * One iterate() of iterators/sirt_vram.py:130-149 with the residual fused
  * into the projector epilogue:  r = b - A x  (one write of r, no copy), then
- * x += M (.) BP_cfg(r).  r is caller-owned scratch with b's shape/dtype.    */
+ * x += M (.) BP_cfg(r).  r is caller-owned scratch with b's shape and dtype;
+ * b and r are each addressed through their own descriptor (any strides).    */
 int32_t ap_sirt_iterate(const ap_config_cb_backproject* bp_cfg,
                         const ap_proj_spec* specs, uint32_t num_projs,
                         const ap_vol_tex_chunk_spec* chunk,
                         const ap_array3* vol_desc, void* x,
                         const ap_array3* m_desc, const void* M,
-                        const ap_array3* sino_desc, const void* b, void* r,
+                        const ap_array3* b_desc, const void* b,
+                        const ap_array3* r_desc, void* r,
                         void* stream);
 
 /* ---- launch-shape tuning -------------------------------------------------

@@ -67,19 +67,37 @@ def test_invalid_arguments_are_reported_not_launched():
     cfg = cbfp.ConfigProject().contents
     chunk = np.array((0.5, (0, 0, 0), (4, 4, 4), (-1, -1, -1)), dtype=_abi.vol_tex_chunk_spec)
     vd, sd = _desc((4, 4, 4)), _desc((3, 5, 5))
-    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), None, 3, _abi.struct_ptr(chunk), vd, FAKE, sd, FAKE, None, None)
+    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), None, 3, _abi.struct_ptr(chunk), vd, FAKE, sd, FAKE, None, None, None)
     assert st == _abi.AP_ERR_INVALID and b"specs" in lib.ap_last_error()
     # view count disagrees with the sinogram
-    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 2, _abi.struct_ptr(chunk), vd, FAKE, sd, FAKE, None, None)
+    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 2, _abi.struct_ptr(chunk), vd, FAKE, sd, FAKE, None, None, None)
     assert st == _abi.AP_ERR_INVALID and b"views" in lib.ap_last_error()
     # chunk larger than the volume
     big = np.array((0.5, (1, 0, 0), (4, 4, 4), (-1, -1, -1)), dtype=_abi.vol_tex_chunk_spec)
-    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 3, _abi.struct_ptr(big), vd, FAKE, sd, FAKE, None, None)
+    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 3, _abi.struct_ptr(big), vd, FAKE, sd, FAKE, None, None, None)
     assert st == _abi.AP_ERR_INVALID and b"chunk" in lib.ap_last_error()
     # bfloat16 sinograms are not supported
     st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE,
-                              _desc((3, 5, 5), _abi.AP_BF16), FAKE, None, None)
+                              _desc((3, 5, 5), _abi.AP_BF16), FAKE, None, None, None)
     assert st == _abi.AP_ERR_UNSUPPORTED
+    # a fused-residual b needs its descriptor, with sino's shape and dtype
+    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, sd, FAKE, None,
+                              FAKE2, None)
+    assert st == _abi.AP_ERR_INVALID and b"descriptor" in lib.ap_last_error()
+    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, sd, FAKE,
+                              _desc((3, 5, 4)), FAKE2, None)
+    assert st == _abi.AP_ERR_INVALID and b"shape" in lib.ap_last_error()
+    st = lib.ap_project_voxel(_abi.struct_ptr(cfg), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, sd, FAKE,
+                              _desc((3, 5, 5), _abi.AP_F16), FAKE2, None)
+    assert st == _abi.AP_ERR_INVALID and b"dtype" in lib.ap_last_error()
+    # the SIRT residual buffer must match the measurements
+    bpf = cbfp.ConfigBackproject(fast_approx=True).contents
+    st = lib.ap_sirt_iterate(_abi.struct_ptr(bpf), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, vd, FAKE, sd, FAKE,
+                             _desc((3, 5, 6)), FAKE2, None)
+    assert st == _abi.AP_ERR_INVALID and b"residual" in lib.ap_last_error()
+    st = lib.ap_sirt_iterate(_abi.struct_ptr(bpf), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, vd, FAKE, sd, FAKE,
+                             sd, FAKE, None)
+    assert st == _abi.AP_ERR_INVALID and b"alias" in lib.ap_last_error()
     # the GBC preset needs a secondary volume of the volume's shape and dtype
     bp = cbfp.ConfigBackproject(backproj_weighting_preset=7, Z_on_val_factor_val=500.0).contents
     st = lib.ap_backproject(_abi.struct_ptr(bp), FAKE, 3, _abi.struct_ptr(chunk), vd, FAKE, None, None, sd, FAKE,

@@ -103,6 +103,50 @@ def test_tilted_adjoint_is_deterministic_and_propagates_nan(device):
     assert hit.any() and np.isnan(got[hit]).all()
 
 
+@pytest.mark.parametrize("kind", ["upright_one_sign", "upright_signed", "tilted_signed"])
+def test_adjoint_wide_dynamic_range(device, kind):
+    # The adjoints sum 64-bit fixed-point terms whose unit is 2^-29 of the
+    # largest |y| over a brick's footprints.  Here the detector rows below
+    # the centre are four decades dimmer than the rows above, so the voxels
+    # of the lower half (in the same bricks as the upper half for the
+    # upright, column kernel) receive only terms near 1e-4 of the unit's
+    # scale.  A biased rounding of the terms (e.g. truncation) shows as a
+    # systematic relative error of ~1e-5 there; round-to-nearest gives
+    # zero-mean errors near 1e-6.
+    rng = np.random.default_rng(77)
+    n, w, nd, V = 32, 0.5, 48, 32
+    if kind.startswith("upright"):
+        pg = ph.circular_geometry(V, nd, nd, 1.0, R=40.0, L=80.0)
+    else:
+        pg = ph.tilted_geometry(V, nd, 1.0, inplane_deg=3.0, outplane_deg=-2.0, offset_uv=(1.5, -1.0),
+                                R=40.0, L=80.0)
+    y = rng.uniform(0.5, 1.0, (V, nd, nd))
+    y[:, :, nd // 2:] *= 1e-4
+    if kind.endswith("signed"):
+        y *= rng.choice([-1.0, 1.0], size=y.shape)
+    y = y.astype(np.float32)
+    cfg = cbfp.ConfigProject()
+    v = Volume(np.zeros((n, n, n), np.float32), voxwid=w)
+    cbfp.backproject_matched(cfg, v, cbfp.ProjectionSet(pg, data=to_gpu(y)), block=True)
+    got = v.as_numpy().astype(np.float64)
+    want = orc.backproject_matched(cfg.contents, pg.to_proj_spec_array(), chunk_for((n, n, n), w),
+                                   np.zeros((n, n, n)), y)
+    top = np.abs(want).max()
+    dim = (np.abs(want) > 1e-7 * top) & (np.abs(want) < 1e-3 * top)
+    err_all = rel_l2(got, want)
+    err_dim = rel_l2(got[dim], want[dim])
+    bias = float((got[dim] - want[dim]).sum() / np.abs(want[dim]).sum())
+    typical = dim & (np.abs(want) >= 0.1 * np.median(np.abs(want[dim])))
+    per_voxel = float(np.max(np.abs(got[typical] - want[typical]) / np.abs(want[typical])))
+    print(f"adjoint dynamic range ({kind}): {dim.sum()} dim voxels, rel L2 all {err_all:.3e}, dim {err_dim:.3e}, "
+          f"relative bias {bias:.2e}, max per-voxel rel error {per_voxel:.3e}")
+    assert dim.sum() > n**3 // 8
+    assert err_all <= 1e-5
+    assert err_dim <= 5e-6
+    assert abs(bias) <= 1e-6
+    assert per_voxel <= 5e-5
+
+
 def _dot_test(pg, n, w, seed):
     import torch
 
@@ -191,6 +235,39 @@ def test_fast_backprojector_detector_edges(device, layout):
     err = rel_l2(v.as_numpy(), want)
     print(f"fast backprojector edges ({layout}) rel L2 {err:.3e}")
     assert (want == 0).any() and np.abs(want).max() > 0   # the field of view ends inside the volume
+    assert err <= 2e-5
+
+
+TILTED_CASES = ["fast_bilinear", "fast_nearest_mask_soft", "fdk_direct_zval", "fast_ss"]
+
+
+@pytest.mark.parametrize("layout", ["f32", "fp16"])
+@pytest.mark.parametrize("name", TILTED_CASES)
+def test_fast_backprojector_tilted_offset_panels(device, name, layout):
+    # the fast branch's general projective map (CB_backproject.cu:478-691,
+    # geometry.py:75-107) on panels tilted in and out of their plane and
+    # shifted along u and v (the batched-solve geometry).  The volume is
+    # wider than the field of view on one side, so some waves have only a
+    # few lanes whose taps leave the detector (both branches of the
+    # edge-free test run) and others have none.
+    import torch
+
+    rng = np.random.default_rng(100 + len(name))
+    shape, w = (30, 26, 22), 0.6
+    pg = ph.tilted_geometry(11, 44, 1.0, inplane_deg=4.0, outplane_deg=-3.0, offset_uv=(3.5, -2.5),
+                            R=40.0, L=80.0)
+    y = rng.uniform(0, 1, (11, 44, 44)).astype(np.float32)
+    data = to_gpu(y)
+    if layout == "fp16":
+        data = data.to(torch.float16)
+        y = data.float().cpu().numpy()
+    cfg = cbfp.ConfigBackproject(**BP_CASES[name])
+    v = Volume(np.zeros(shape, np.float32), voxwid=w)
+    cbfp.backproject(cfg, v, cbfp.ProjectionSet(pg, data=data), block=True)
+    want = orc.backproject(cfg.contents, pg.to_proj_spec_array(), chunk_for(shape, w), np.zeros(shape), y)
+    err = rel_l2(v.as_numpy(), want)
+    print(f"tilted backproject[{name}, {layout}] rel L2: {err:.3e}")
+    assert np.abs(want).max() > 0
     assert err <= 2e-5
 
 

@@ -86,7 +86,7 @@ def test_forward_chunk_fortran_order_and_fused_residual(device):
     cfg = cbfp.ConfigProject().contents
     _abi.check(_abi.lib().ap_project_voxel(_abi.struct_ptr(cfg), _abi.ptr(sp), 5, _abi.struct_ptr(ch),
                                            _abi.array3(vt), _abi.ptr(vt), _abi.array3(out), _abi.ptr(out),
-                                           None, None))
+                                           None, None, None))
     torch.cuda.synchronize()
     want = orc.project_voxel(cfg, specs, ch, vol, np.zeros((5, 40, 33)))
     assert rel_l2(out.cpu().numpy(), want) <= 1e-5
@@ -97,7 +97,7 @@ def test_forward_chunk_fortran_order_and_fused_residual(device):
     ch0 = chunk_for(shape, w)
     _abi.check(_abi.lib().ap_project_voxel(_abi.struct_ptr(fp), _abi.ptr(sp), 5, _abi.struct_ptr(ch0),
                                            _abi.array3(vt), _abi.ptr(vt), _abi.array3(r), _abi.ptr(r),
-                                           _abi.ptr(b), None))
+                                           _abi.array3(b), _abi.ptr(b), None))
     torch.cuda.synchronize()
     np.testing.assert_allclose(r.cpu().numpy(), b.cpu().numpy() - c, rtol=1e-5, atol=1e-4)
 

@@ -1,0 +1,241 @@
+"""GPU parity of the iterative solvers with compositions of the FP64 oracle.
+
+- Landweber (landweber_vram.py:127-260): the power-iteration Rayleigh
+  quotient against the same iteration on dense FP64 matrices of A and B
+  built column by column from the oracle (<= 1e-5), and against the largest
+  eigenvalue of B A from numpy.linalg.eigvalsh (B = M A^T with the cylinder
+  mask M for the exact backprojector, which the dense matrices confirm);
+  three iterations with a detector zero-mask against the oracle composition.
+- SIRT and Landweber with measurements given as a strided view: bit-identical
+  to the same solve on a contiguous copy (the residual buffer and the
+  measurements are addressed through their own descriptors).
+- Batched fp16 SIRT (one HIP graph over tilted-detector problems): each
+  volume after 3 iterations against an FP64 oracle composition that rounds to
+  fp16 wherever the GPU path stores fp16 (measurements, preconditioner steps,
+  residual, volume); tolerance 5e-4 relative L2 (a flipped fp16 rounding
+  moves an element by 2^-11).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from aprict_amd import phantoms as ph
+from aprict_amd.cb import flatpanel as cbfp
+from aprict_amd.iterators import BatchedSIRT, Landweber_flatpanel, SIRT_flatpanel
+from aprict_amd.volume import Volume
+from helpers import chunk_for, rel_l2
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def to_gpu(a, dtype=None):
+    import torch
+
+    t = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    return t if dtype is None else t.to(dtype)
+
+
+def _cyl_mask(n0, n1, n2, w, rad):
+    """1 where the voxel centre lies inside the cylinder of radius rad about z."""
+    i = (np.arange(n0) - (n0 - 1) / 2) * w
+    j = (np.arange(n1) - (n1 - 1) / 2) * w
+    inside = (i[:, None] ** 2 + j[None, :] ** 2) <= rad**2
+    return np.broadcast_to(inside[:, :, None], (n0, n1, n2)).astype(np.float64)
+
+
+def test_landweber_power_iteration_matches_dense_matrices(device):
+    n, w = 8, 1.0
+    pg = ph.circular_geometry(6, 20, 20, 1.0, R=30.0, L=60.0)
+    specs, ch = pg.to_proj_spec_array(), chunk_for((n, n, n), w)
+    fp = cbfp.ConfigProject().contents
+    cfg = Landweber_flatpanel.Config(fast_backproject=False, rayleigh_quotient_num_iterations=40,
+                                     bp_soft_from_detedge_relrad=None, soft_from_cylinder_relrad=None)
+    bp = cbfp.ConfigBackproject(fast_approx=False, cyl_mask_rad=n / 2 * w).contents
+    nvox, npix = n**3, 6 * 20 * 20
+    A = np.zeros((npix, nvox))
+    for i in range(nvox):
+        e = np.zeros(nvox)
+        e[i] = 1.0
+        A[:, i] = orc.project_voxel(fp, specs, ch, e.reshape(n, n, n), np.zeros((6, 20, 20))).ravel()
+    B = np.zeros((nvox, npix))
+    for q in range(npix):
+        e = np.zeros(npix)
+        e[q] = 1.0
+        B[:, q] = orc.backproject(bp, specs, ch, np.zeros((n, n, n)), e.reshape(6, 20, 20)).ravel()
+    # the exact backprojector is the transpose of the forward, masked to the cylinder
+    mask = _cyl_mask(n, n, n, w, n / 2 * w).ravel()
+    np.testing.assert_allclose(B, mask[:, None] * A.T, rtol=1e-12, atol=1e-12 * np.abs(A).max())
+    inside = mask > 0
+    Ac = A[:, inside]
+    lam_max = float(np.linalg.eigvalsh(Ac.T @ Ac)[-1])
+    # the power iteration of landweber_vram.py:127-209 on the dense matrices,
+    # from the same random start
+    np.random.seed(123)  # noqa: NPY002
+    v = np.abs(np.random.randn(n, n, n) * (1.0 / n)).astype(np.float32).astype(np.float64).ravel()  # noqa: NPY002
+    norm = np.linalg.norm(v)
+    rq = 0.0
+    for _ in range(cfg.rayleigh_quotient_num_iterations):
+        v = v / norm
+        y = A @ v
+        rq = float(y @ y)
+        v = B @ y
+        norm = np.linalg.norm(v)
+    np.random.seed(123)  # noqa: NPY002
+    lw = Landweber_flatpanel(cfg, Volume.zeros((n, n, n), voxwid=w),
+                             cbfp.ProjectionSet(pg, data=to_gpu(np.ones((6, 20, 20)))))
+    got = lw.rayleigh_quotients[0]
+    print(f"Landweber lambda: GPU {got:.8e}, dense power iteration {rq:.8e}, eigvalsh {lam_max:.8e}")
+    assert got == pytest.approx(rq, rel=1e-5)
+    assert got == pytest.approx(lam_max, rel=1e-3)
+
+
+def _small_problem():
+    pg = ph.circular_geometry(30, 48, 40, 1.0, R=60.0, L=120.0)
+    n, w = 24, 0.8
+    shapes = ph.random_ellipsoids(6, n * w / 2, seed=7)
+    truth = orc.voxelise_forbild(shapes, chunk_for((n, n, n), w), np.zeros((n, n, n)))
+    b = orc.project_voxel(cbfp.ConfigProject().contents, pg.to_proj_spec_array(), chunk_for((n, n, n), w), truth,
+                          np.zeros((30, 48, 40)))
+    return pg, n, w, b
+
+
+@pytest.mark.parametrize("fast", [False, True], ids=["exact", "fast"])
+def test_landweber_iterations_match_oracle_composition(device, fast):
+    # x <- x + (accel / lambda) B (z (.) (b - A x)), z a detector zero-mask
+    # (landweber_vram.py:221-260); lambda is the solver's own (pinned above).
+    # Tolerance: twice the deviation of the same composition run with the
+    # FP32 build of the oracle (the detector-edge softening of this small
+    # problem amplifies FP32 rounding), at least 1e-5.
+    pg, n, w, b = _small_problem()
+    rng = np.random.default_rng(9)
+    z = (rng.uniform(0, 1, (48, 40)) > 0.1).astype(np.float32)
+    cfg = Landweber_flatpanel.Config(fast_backproject=fast, rayleigh_quotient_num_iterations=8,
+                                     acceleration_factor=1.5)
+    np.random.seed(5)  # noqa: NPY002
+    vol = Volume.zeros((n, n, n), voxwid=w)
+    lw = Landweber_flatpanel(cfg, vol, cbfp.ProjectionSet(pg, data=to_gpu(b)), proj_zero_masks=to_gpu(z)[None])
+    lam = lw.rayleigh_quotients[0]
+    specs, ch = pg.to_proj_spec_array(), chunk_for((n, n, n), w)
+    fp = cbfp.ConfigProject().contents
+    bp = cbfp.ConfigBackproject(fast_approx=fast, cyl_mask_rad=n / 2 * w, soften_det_edges_proportion=0.05).contents
+    bz = b.astype(np.float32).astype(np.float64) * z
+    x = np.zeros((n, n, n))
+    x32 = np.zeros((n, n, n), np.float32)
+    for it in range(3):
+        r = z * (bz - orc.project_voxel(fp, specs, ch, x, np.zeros(b.shape)))
+        x = orc.backproject(bp, specs, ch, x, r * (1.5 / lam))
+        r32 = (z * (bz - orc.project_voxel(fp, specs, ch, x32, np.zeros(b.shape), fp32=True))).astype(np.float32)
+        x32 = orc.backproject(bp, specs, ch, x32, (r32 * np.float32(1.5 / lam)).astype(np.float32), fp32=True)
+        lw.iterate()
+        err = rel_l2(vol.as_numpy(), x)
+        tol = max(2 * rel_l2(x32, x), 1e-5)
+        print(f"Landweber ({'fast' if fast else 'exact'}) iteration {it + 1}: rel L2 vs oracle {err:.3e} "
+              f"(tol {tol:.2e})")
+        assert err <= tol
+
+
+def test_solvers_accept_strided_measurements(device):
+    import torch
+
+    pg2 = ph.circular_geometry(24, 40, 36, 1.0, R=60.0, L=120.0)
+    pg = pg2.sliced(slice(0, None, 2))
+    n, w = 20, 0.8
+    shapes = ph.random_ellipsoids(5, n * w / 2, seed=3)
+    truth = orc.voxelise_forbild(shapes, chunk_for((n, n, n), w), np.zeros((n, n, n)))
+    full = orc.project_voxel(cbfp.ConfigProject().contents, pg2.to_proj_spec_array(), chunk_for((n, n, n), w),
+                             truth, np.zeros((24, 40, 36)))
+    data = to_gpu(full)
+    strided = data[::2]
+    assert not strided.is_contiguous()
+    results = []
+    for b in (strided, strided.contiguous()):
+        v = Volume.zeros((n, n, n), voxwid=w)
+        s = SIRT_flatpanel(SIRT_flatpanel.Config(), v, cbfp.ProjectionSet(pg, data=b))
+        for _ in range(2):
+            s.iterate()
+        np.random.seed(8)  # noqa: NPY002
+        v2 = Volume.zeros((n, n, n), voxwid=w)
+        lw = Landweber_flatpanel(Landweber_flatpanel.Config(rayleigh_quotient_num_iterations=4), v2,
+                                 cbfp.ProjectionSet(pg, data=b))
+        for _ in range(2):
+            lw.iterate()
+        torch.cuda.synchronize()
+        results.append((v.as_numpy(), s.residual.cpu().numpy(), v2.as_numpy()))
+    for a, c in zip(results[0], results[1], strict=True):
+        np.testing.assert_array_equal(a, c)
+    assert np.abs(results[0][0]).max() > 0 and np.abs(results[0][2]).max() > 0
+
+
+# ------------------------------------------------------------ batched fp16
+def _f16(a):
+    return np.asarray(a, dtype=np.float64).astype(np.float16).astype(np.float64)
+
+
+def _softmask(a, r0):
+    nx, ny = a.shape[:2]
+    i = np.arange(nx)[:, None] - (nx - 1) / 2
+    j = np.arange(ny)[None, :] - (ny - 1) / 2
+    rr = np.sqrt(i**2 + j**2) / min(nx, ny) * 2
+    m = np.ones_like(rr)
+    out = rr >= r0
+    if r0 >= 1:
+        m[out] = 0
+    else:
+        t = np.minimum((rr - r0) / (1 - r0), 1)
+        m[out] = np.clip((1 + np.cos(np.pi * t[out])) / 2, 0, 1)
+    return a * m[:, :, None]
+
+
+def _sirt_fp16_oracle(pg, n, w, b16, iters):
+    """SIRT_flatpanel with fp16 storage, composed from FP64 oracle operators
+    and rounded to fp16 at every point where the GPU path stores a value."""
+    specs, ch = pg.to_proj_spec_array(), chunk_for((n, n, n), w)
+    fp = cbfp.ConfigProject().contents
+    cyl = n / 2 * w
+    bp = cbfp.ConfigBackproject(fast_approx=True, cyl_mask_rad=cyl, soften_det_edges_proportion=0.05).contents
+
+    def A(x):  # noqa: N802
+        return orc.project_voxel(fp, specs, ch, x, np.zeros(b16.shape))
+
+    def B(y):  # noqa: N802
+        return orc.backproject(bp, specs, ch, np.zeros((n, n, n)), y)
+
+    s = 1.0 / n
+    M = _f16(_softmask(np.full((n, n, n), _f16(s)), 1.0))
+    p = _f16(A(M))
+    mean = p.mean()
+    p = _f16(p * (s / mean))
+    s /= mean / s
+    M = B(p)
+    M = _f16(M)
+    nz = M != 0
+    M[nz] = _f16(1 / (M[nz] + 0.1 * np.exp(-M[nz] / 0.1)))
+    M = _f16(M * s)
+    M = _f16(_softmask(M, 0.95))
+    x = np.zeros((n, n, n))
+    for _ in range(iters):
+        r = _f16(b16 - A(x))
+        x = _f16(x + M * B(r))
+    return x
+
+
+def test_batched_fp16_sirt_matches_oracle_composition(device):
+    import torch
+
+    entries = ph.batched_problems(3, 17, n_range=(20, 32), views_range=(16, 28), det_range=(32, 56))
+    data = [ph.batched_problem_data(e, "float16")[:2] for e in entries]
+    b16s = [meas.data.double().cpu().numpy() for _, meas in data]
+    batched = BatchedSIRT(data, streams=2)
+    batched.capture(3)
+    batched.run()
+    torch.cuda.synchronize()
+    for e, b16, vol in zip(entries, b16s, batched.volumes, strict=True):
+        want = _sirt_fp16_oracle(e["geometry"], e["N"], e["voxwid"], b16, 3)
+        err = rel_l2(vol.as_numpy(), want)
+        print(f"batched fp16 SIRT N={e['N']} V={e['views']} det={e['det']} tilt={e['tilt_deg']}: "
+              f"rel L2 vs fp16-rounded oracle composition {err:.3e}")
+        assert np.abs(want).max() > 0
+        assert err <= 5e-4
