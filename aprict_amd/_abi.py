@@ -152,7 +152,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ap_struct_size": (_I32, [_I32]),
     "ap_project_voxel": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _P]),
     "ap_backproject": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _A3, _P, _P]),
-    "ap_backproject_matched": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _F32, _P]),
+    "ap_backproject_matched": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _F32, _P, ctypes.c_uint64, _P]),
+    "ap_backproject_matched_workspace_size": (ctypes.c_uint64, [_P, _U32, _A3]),
     "ap_project_forbild": (_I32, [_P, _P, _U32, _P, _U32, _A3, _P, _P, _P]),
     "ap_voxelise_forbild": (_I32, [_P, _U32, _P, _A3, _P, _U32, _P]),
     "ap_forbild_bboxes": (_I32, [_P, _U32, _P, _P]),
@@ -168,6 +169,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ap_volume_softmask_cylinder": (_I32, [_A3, _P, _F32, _P]),
     "ap_array_dot": (_I32, [_A3, _P, _A3, _P, _P, _P]),
     "ap_tv_gradient": (_I32, [_A3, _P, _A3, _P, _F32, _F32, _P]),
+    "ap_tv_value": (_I32, [_A3, _P, _F32, _P, _P]),
     "ap_sirt_iterate": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _A3, _P, _P]),
     "ap_tune_count": (_I32, [_I32]),
     "ap_tune_get": (_I32, [_I32]),
@@ -290,3 +292,39 @@ def stream_handle(device=None) -> ctypes.c_void_p:
 def struct_ptr(arr: np.ndarray) -> ctypes.c_void_p:
     """Pointer to a numpy structured scalar/array kept alive by the caller."""
     return ctypes.c_void_p(arr.ctypes.data)
+
+
+# --------------------------------------------------------------------------
+# workspaces
+# --------------------------------------------------------------------------
+_WORKSPACES: dict[tuple[int, int], object] = {}
+_WS_LOCK = threading.Lock()
+
+
+def workspace(nbytes: int, device=None):
+    """Device scratch of at least ``nbytes`` bytes for the current stream of
+    ``device`` (a cached uint8 tensor, grown on demand; torch allocations are
+    512-byte aligned).  Calls that share it are ordered by their stream."""
+    torch = _torch()
+    if nbytes <= 0:
+        return None
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    key = (idx, int(torch.cuda.current_stream(idx).cuda_stream))
+    with _WS_LOCK:
+        buf = _WORKSPACES.get(key)
+        if buf is None or buf.numel() < nbytes:
+            _WORKSPACES.pop(key, None)
+            buf = torch.empty(int(nbytes), dtype=torch.uint8, device=torch.device("cuda", idx))
+            _WORKSPACES[key] = buf
+        return buf
+
+
+def matched_workspace(cfg, num_projs: int, sino) -> tuple[ctypes.c_void_p, int]:
+    """(pointer, bytes) of the workspace that lets ap_backproject_matched take
+    every upright view of the call in one pass (NULL, 0 when it needs none)."""
+    nbytes = int(lib().ap_backproject_matched_workspace_size(struct_ptr(cfg), int(num_projs), array3(sino)))
+    if nbytes == 0:
+        return ctypes.c_void_p(0), 0
+    buf = workspace(nbytes, sino.device)
+    return ptr(buf), nbytes

@@ -40,6 +40,10 @@ class Projector:
         self._chunk = np.array((self.voxwid, (0, 0, 0), self.vol_shape, tuple(texpos)),
                                dtype=_abi.vol_tex_chunk_spec)
         self._cfg = self.config.contents
+        # the same operator with the sign of multiply_before_adding flipped:
+        # A_residual's fused epilogue writes b - A x
+        self._cfg_neg = self.config.contents
+        self._cfg_neg["multiply_before_adding"] = -self._cfg_neg["multiply_before_adding"]
 
     def A(self, x, out=None, b=None):  # noqa: N802
         """out = (b or 0) + A x (out float32, allocated if None)."""
@@ -57,6 +61,22 @@ class Projector:
             _abi.stream_handle(x.device)), "A")
         return out
 
+    def A_residual(self, x, b, out=None):  # noqa: N802
+        """out = b - A x in one pass (the projector's fused residual epilogue)."""
+        torch = _torch()
+        if tuple(x.shape) != self.vol_shape or tuple(b.shape) != self.sino_shape:
+            msg = f"shapes {tuple(x.shape)}, {tuple(b.shape)} != {self.vol_shape}, {self.sino_shape}"
+            raise ValueError(msg)
+        _abi.require_device(x, "volume")
+        _abi.require_device(b, "measurements")
+        if out is None:
+            out = torch.empty(self.sino_shape, dtype=torch.float32, device=x.device)
+        _abi.check(_abi.lib().ap_project_voxel(
+            _abi.struct_ptr(self._cfg_neg), _abi.ptr(self._specs), self.sino_shape[0], _abi.struct_ptr(self._chunk),
+            _abi.array3(x), _abi.ptr(x), _abi.array3(out), _abi.ptr(out), _abi.array3(b), _abi.ptr(b),
+            _abi.stream_handle(x.device)), "A_residual")
+        return out
+
     def AT(self, y, out=None, alpha: float = 1.0):  # noqa: N802
         """out += alpha A^T y (out float32 zeros if None)."""
         torch = _torch()
@@ -66,9 +86,10 @@ class Projector:
         _abi.require_device(y, "sinogram")
         if out is None:
             out = torch.zeros(self.vol_shape, dtype=torch.float32, device=y.device)
+        ws, wsb = _abi.matched_workspace(self._cfg, self.sino_shape[0], y)
         _abi.check(_abi.lib().ap_backproject_matched(
             _abi.struct_ptr(self._cfg), _abi.ptr(self._specs), self.sino_shape[0], _abi.struct_ptr(self._chunk),
-            _abi.array3(out), _abi.ptr(out), None, None, _abi.array3(y), _abi.ptr(y), float(alpha),
+            _abi.array3(out), _abi.ptr(out), None, None, _abi.array3(y), _abi.ptr(y), float(alpha), ws, wsb,
             _abi.stream_handle(y.device)), "AT")
         return out
 
@@ -99,6 +120,15 @@ def project(x, projector: Projector):
     if _FN is None:
         _FN = _make_function()
     return _FN.apply(x, projector)
+
+
+def tv_value(x, eps: float = 1e-3) -> float:
+    """sum_i sqrt(|grad_fwd x_i|^2 + eps^2) accumulated in FP64 (HIP kernel)."""
+    torch = _torch()
+    out = torch.empty(1, dtype=torch.float64, device=x.device)
+    _abi.check(_abi.lib().ap_tv_value(_abi.array3(x), _abi.ptr(x), float(eps), _abi.ptr(out),
+                                      _abi.stream_handle(x.device)), "tv_value")
+    return float(out.item())
 
 
 def tv_gradient(x, weight: float, eps: float = 1e-3, out=None):

@@ -553,10 +553,11 @@ def backproject_matched(config: ConfigProject, volume: Volume, projs: Projection
     md = _abi.array3(mdat) if mdat is not None else None
     cfg = config.contents
     chunk = _chunk_of(volume)
+    ws, wsb = _abi.matched_workspace(cfg, data.shape[0], data)
     _abi.check(_abi.lib().ap_backproject_matched(
         _abi.struct_ptr(cfg), _abi.ptr(projs.specs_device()), data.shape[0], _abi.struct_ptr(chunk),
         _abi.array3(vdat), _abi.ptr(vdat), md, _abi.ptr(mdat), _abi.array3(data), _abi.ptr(data),
-        float(alpha), _abi.stream_handle(data.device)), "backproject_matched")
+        float(alpha), ws, wsb, _abi.stream_handle(data.device)), "backproject_matched")
     if block:
         torch.cuda.synchronize(data.device)
     return volume

@@ -77,18 +77,6 @@ inline int64_t array3_numel(const ap_array3* d) {
 int32_t ap_check_array3(const ap_array3* d, const void* ptr, const char* name,
                         bool allow_f16, bool allow_bf16, bool allow_f64);
 
-// Unit 2^b of the 64-bit fixed-point brick sums of the matched adjoints.  A
-// term is w * y * scale * 2^b / max|y * scale| with |w| <= sqrt(3) < 2 (voxel
-// units), so below 2^(b+1); at most `terms` = views x rays per view terms
-// reach one voxel (a ray crosses a voxel once), so b <= 62 - log2(terms)
-// keeps every sum below 2^63.  b = 29 unless the problem is larger than
-// 2^33 terms; 0 (unsupported) when fewer than 12 bits would remain.
-inline float adjoint_fixed_point_unit(double terms) {
-    int b = 29;
-    while (b >= 12 && terms * (double)(2ull << b) > 9223372036854775808.0) b--;
-    return b < 12 ? 0.f : (float)(1ull << b);
-}
-
 // ---------------------------------------------------------------------------
 // dispatch a functor over a storage dtype
 // ---------------------------------------------------------------------------

@@ -1,0 +1,658 @@
+// gather.hip -- matched adjoint of the Siddon forward projector for views
+// whose detector v axis is parallel to z and whose u axis has no z component
+// (circular and helical scans with upright panels), computed as a gather over
+// voxels: no atomics, every voxel sum accumulated in FP64 registers.
+//
+// Geometry.  For such a view all rays through detector (sub-)column q lie in
+// one vertical plane through the source: they share their x-y line, and
+// along that line, parametrised by the horizontal distance h from the source,
+// ray t has height z_t(h) = sz + h * dz_t / l, where dz_t is the height of
+// its detector point above the source and l the horizontal source-pixel
+// distance.  Order the rays by height (dz_t = dz0 + t dzs, dzs > 0).  The
+// column's line crosses x-y cell (i, j) over [h_a, h_b]; the Siddon length of
+// ray t in voxel (i, j, k) is sec_t times the measure of the h in [h_a, h_b]
+// where k <= z_t(h) < k + 1.  With
+//     F_t(k) = |{h in [h_a, h_b] : z_t(h) < k}|   and
+//     G(k)   = sum_t yhat_t F_t(k),   yhat_t = y_t sec_t,
+// the adjoint's contribution of the column to voxel k is G(k+1) - G(k).
+// Rays entirely below plane k contribute h_b - h_a each, so
+//     G(k) = (h_b - h_a) P[tf] + F_tf(k) yhat_tf,
+// where P is the exclusive prefix sum of yhat over t and tf the first ray
+// that is not entirely below k.  The next rays are entirely above k when the
+// rays of a column are further apart in z than one of them climbs within a
+// cell (kappa below); (cell, column) pairs where that is not certain add
+// F_t(k) yhat_t of every ray that may straddle the plane.  tf is the integer
+// part of a linear function kappa of k, P is read from LDS, and F_tf(k)
+// follows from the crossing of plane k by ray tf.
+//
+// A pre-pass (k_gather_prep) writes, per view and sub-column, the FP64
+// prefix sums P of the sec-weighted sinogram column (in z order), the
+// column's x-y crossing constants (the forward's col_setup, so the cell
+// boundaries along h are the forward's own FP32 values) and, per view, the
+// reciprocal heights 1/dz_t.  The main kernel (k_gather_adjoint) gives each
+// workgroup a tile of x-y cells and a z range of 64*kJ boundaries; each wave
+// owns kC cells, each lane one boundary per 64 in z.  Per view the workgroup
+// stages the prefix window of the columns crossing the tile into LDS, each
+// wave derives its (cell, column) pairs one per lane, then for every pair
+// every lane evaluates G at its boundaries (about 20 VALU operations, two
+// LDS reads) and adds it to its FP64 accumulators.  The voxel values are
+// differences of neighbouring boundaries, scaled and added to the volume
+// once at the end.
+//
+// Precision: the sums P and the accumulators are FP64, so a voxel's value is
+// exact to the FP32 rounding of its own terms whatever the dynamic range of
+// the sinogram around it; the crossing of plane k by ray tf is formed in FP64
+// relative to the column's closest approach (FP32 error of the forward's own
+// crossing parameters).  A ray lying in a z plane (the forward's horizontal
+// rows, |slope| < 1e-20) belongs to the voxel below the plane, as in the
+// forward; such rays are kept out of P and added separately.
+#include "column.h"
+#include "gather.h"
+
+namespace {
+
+constexpr int kXcdG = 8;
+constexpr int kGCap = 4608;        // FP64 entries of the LDS prefix window (36 KB)
+constexpr int kGMaxCols = 16;      // columns staged at once
+constexpr int kGRowMax = 2048;     // rows of the window (1/dz table)
+
+// per-(view, sub-column) record
+struct GCol {
+    float r[2], c[2], P0[2];   // x-y crossing parameters along h - hc (col_setup)
+    int32_t sg[2], ifix[2];
+    float hs, hd;              // source and pixel along h - hc (hd <= hs: no ray meets the chunk)
+    float l, hc;               // horizontal source-pixel distance, closest approach to the chunk centre
+    double gam;                // hc / l
+    float yh;                  // value of the column's rays lying in a z plane (sec = 1), 0 if none
+    int32_t ifixh;             // the voxel row they belong to
+    float lods;                // l / dzs
+    float pad;
+};
+static_assert(sizeof(GCol) == 80, "GCol layout");
+
+__host__ __device__ inline size_t g_align(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Rows past the last ray are padded (P: the total, 1/dz: 0) far enough that
+// the window loads (whole 256-byte chunks from a row <= NT + 1) stay inside
+// the row: kGPad >= 32 + 2.
+constexpr uint32_t kGPad = 40;
+
+struct GLayout {
+    size_t views, count, cols, pref, iz, total;
+    uint32_t cs;   // entries per column of P and per view of 1/dz: NT + kGPad
+};
+
+__host__ __device__ inline GLayout g_layout(uint32_t V, uint32_t NQ, uint32_t NT) {
+    GLayout L;
+    L.cs = NT + kGPad;
+    L.views = 0;
+    L.count = g_align((size_t)V * sizeof(GView));
+    L.cols = L.count + 256;
+    // one spare record: the record loads copy whole 256-byte chunks
+    L.pref = L.cols + g_align(((size_t)V * NQ + 4) * sizeof(GCol));
+    L.iz = L.pref + g_align((size_t)V * NQ * L.cs * sizeof(double));
+    L.total = L.iz + g_align((size_t)V * L.cs * sizeof(double));
+    return L;
+}
+
+struct GCfg {
+    uint32_t V, nu, nv, ssu, ssv, NQ, NT, cs;
+    int64_t s0, s1, s2;   // sinogram element strides
+};
+
+// ---------------------------------------------------------------------------
+// pre-pass: one wave per (view, sub-column)
+// ---------------------------------------------------------------------------
+template <typename Tp>
+__global__ __launch_bounds__(256) void k_gather_prep(RayCfg rc, GCfg gc, const ap_proj_spec* __restrict__ specs,
+                                                     const Tp* __restrict__ sino, GView* __restrict__ gviews,
+                                                     GCol* __restrict__ gcols, double* __restrict__ pref,
+                                                     double* __restrict__ izt) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t w = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= (uint64_t)gc.V * gc.NQ) return;
+    const uint32_t p = (uint32_t)(w / gc.NQ), q = (uint32_t)(w % gc.NQ);
+    GView gv;
+    const bool ok = gather_view(rc, specs, p, gc.ssu, gc.ssv, gc.NT, gv);
+    if (q == 0 && lane == 0) gviews[p] = gv;
+    if (!ok) return;
+    const ViewRays vr = make_view_rays(rc, specs, p);
+    const uint32_t j = q / gc.ssu;
+    const float ju = (float)j + sub_offset_g(q % gc.ssu, gc.ssu);
+    Col2D C;
+    const bool hit = col_setup(vr, rc.lo, rc.hi, ju, C);
+    double* P = pref + (size_t)w * gc.cs;
+    const uint32_t NT = gc.NT;
+    double carry = 0.0, yh = 0.0;
+    int ifixh = INT32_MIN;
+    // rows in z order, 64 at a time (coalesced): inclusive wave scan plus the
+    // running total of the previous chunks; P[t] is the exclusive prefix
+    for (uint32_t t0 = 0; t0 < NT + kGPad; t0 += 64) {
+        const uint32_t t = t0 + (uint32_t)lane;
+        double yv = 0.0;
+        if (hit && t < NT) {
+            const uint32_t sr = gv.flip ? NT - 1 - t : t;
+            const uint32_t k = sr / gc.ssv;
+            const float kv = (float)k + sub_offset_g(sr % gc.ssv, gc.ssv);
+            const double m = fma((double)kv, C.z.dm, C.z.m0);
+            const float mf = (float)m;
+            const float y = to_f32(sino[(int64_t)p * gc.s0 + (int64_t)j * gc.s1 + (int64_t)k * gc.s2]);
+            if (fabsf(mf) < 1e-20f) {
+                // lies in its z plane: the forward's horizontal row (lane_setup)
+                yh += (double)y;
+                ifixh = (int)ceil(fma(C.z.hc, m, C.z.zc)) - 1;
+            } else {
+                yv = (double)(y * __builtin_amdgcn_sqrtf(__builtin_fmaf(mf, mf, 1.f)));
+            }
+        }
+        double incl = yv;
+        for (int o = 1; o < 64; o <<= 1) {
+            const double v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        if (t < gc.cs) P[t] = carry + incl - yv;
+        carry += __shfl(incl, 63, 64);
+    }
+    // the horizontal rows of the column (at most one in practice; summed)
+    for (int o = 32; o > 0; o >>= 1) {
+        yh += __shfl_xor(yh, o, 64);
+        ifixh = max(ifixh, __shfl_xor(ifixh, o, 64));
+    }
+    if (lane == 0) {
+        GCol g;
+        for (int a = 0; a < 2; a++) {
+            g.r[a] = hit ? C.r[a] : 0.f;
+            g.c[a] = hit ? C.c[a] : 0.f;
+            g.P0[a] = hit ? C.P0[a] : 0.f;
+            g.sg[a] = hit ? C.sg[a] : 0;
+            g.ifix[a] = hit ? C.ifix[a] : 0;
+        }
+        g.hs = hit ? C.hs : 0.f;
+        g.hd = hit ? C.hd : 0.f;   // hd <= hs: empty
+        const double qx = vr.Q00[0] + (double)ju * vr.U[0] - vr.S[0];
+        const double qy = vr.Q00[1] + (double)ju * vr.U[1] - vr.S[1];
+        const double l = hit ? sqrt(qx * qx + qy * qy) : 1.0;
+        g.l = (float)l;
+        g.hc = hit ? (float)C.z.hc : 0.f;
+        g.gam = hit ? C.z.hc / l : 0.0;
+        g.yh = (float)yh;
+        g.ifixh = ifixh == INT32_MIN ? 0 : ifixh;
+        g.lods = (float)(l / (double)gv.dzs);
+        g.pad = 0.f;
+        gcols[w] = g;
+    }
+    // 1/dz per row (0 past the last ray), once per view
+    if (q == 0) {
+        for (uint32_t t = (uint32_t)lane; t < gc.cs; t += 64) {
+            double v = 0.0;
+            if (t < NT) {
+                const uint32_t sr = gv.flip ? NT - 1 - t : t;
+                const float kv = (float)(sr / gc.ssv) + sub_offset_g(sr % gc.ssv, gc.ssv);
+                const double dz = vr.Q00[2] + (double)kv * vr.V[2] - vr.S[2];
+                v = dz != 0.0 ? 1.0 / dz : 0.0;
+            }
+            izt[(size_t)p * gc.cs + t] = v;
+        }
+    }
+}
+
+// number of views the gather kernel does not take (for the general adjoint's early exit)
+__global__ __launch_bounds__(256) void k_gather_count(const GView* __restrict__ gviews, uint32_t V,
+                                                      uint32_t* __restrict__ left) {
+    __shared__ uint32_t part[4];
+    uint32_t n = 0;
+    for (uint32_t p = threadIdx.x; p < V; p += 256) n += gviews[p].ok ? 0u : 1u;
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) *left = part[0] + part[1] + part[2] + part[3];
+}
+
+// ---------------------------------------------------------------------------
+// main kernel
+// ---------------------------------------------------------------------------
+struct GTile {
+    uint32_t ntx, nty, ntz, total, per_xcd;
+};
+
+// integer part of x, one instruction (x within int range)
+__device__ __forceinline__ int cvt_flr(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// prefix window of one view for the tile (views taken 64 at a time, one per lane)
+struct GWin {
+    int qlo, qhi, tlo, W;   // columns [qlo, qhi], rows tlo .. tlo + W - 1 of P (qlo > qhi: nothing)
+    float kc;                // kappa offset: 1 - tlo - dz0 / dzs
+    int ncg;                 // columns per staging group
+    int pad[2];
+};
+
+template <typename Tv, typename Tm, int kW, int kC, int kJ, int kTX>
+__global__ __launch_bounds__(kW * 64, 4) void k_gather_adjoint(
+    RayCfg rc, GCfg gc, GTile tl, const GView* __restrict__ gviews, const GCol* __restrict__ gcols,
+    const double* __restrict__ pref, const double* __restrict__ izt, double scale, Tv* __restrict__ vol,
+    int64_t vsx, int64_t vsy, int64_t vsz, const Tm* __restrict__ mult, int64_t msx, int64_t msy, int64_t msz) {
+    constexpr int kCells = kW * kC;
+    constexpr int kTY = kCells / kTX;
+    constexpr int kT = kW * 64;
+    static_assert(kTX * kTY == kCells, "tile shape");
+    static_assert(64 / kC >= kGMaxCols, "one setup pass per group");
+    __shared__ double Pw[kGCap];
+    __shared__ double Iw[kGRowMax];
+    __shared__ __align__(16) GCol Cw[kGMaxCols];   // 16 records = 5 whole chunks
+    __shared__ float Qc[(kTX + 1) * (kTY + 1)];
+    __shared__ GWin Win[64];
+    // per-wave (cell, column) pair records of the setup pass, one per lane:
+    // h_a, h_b - h_a, alpha_a, alpha_b, l, window offset | flags, hc / l (FP64)
+    __shared__ __align__(16) float Prec[kW][64][8];
+    const uint32_t L = blockIdx.x;
+    const uint32_t logical = (L % kXcdG) * tl.per_xcd + L / kXcdG;
+    if (logical >= tl.total) return;
+    const uint32_t tx = logical % tl.ntx, ty = (logical / tl.ntx) % tl.nty, tz = logical / (tl.ntx * tl.nty);
+    const int X0 = rc.lo[0] + (int)tx * kTX, Y0 = rc.lo[1] + (int)ty * kTY;
+    const int X1 = min(X0 + kTX, rc.hi[0]), Y1 = min(Y0 + kTY, rc.hi[1]);
+    const int z0 = rc.lo[2] + (int)tz * 64 * kJ;
+    const int z1 = min(z0 + 64 * kJ, rc.hi[2]);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int NT = (int)gc.NT;
+    // this lane's setup slot: cell si of the wave, column ss of the cell
+    const int si = lane % kC, ss = lane / kC;
+    int cgi[kC], cgj[kC];
+    bool cval[kC];
+#pragma unroll
+    for (int i = 0; i < kC; i++) {
+        const int c = wave + kW * i;
+        cgi[i] = X0 + c % kTX;
+        cgj[i] = Y0 + c / kTX;
+        cval[i] = cgi[i] < X1 && cgj[i] < Y1;
+    }
+    int s_ci = 0, s_cj = 0, s_cl = 0;
+    bool s_cv = false;
+#pragma unroll
+    for (int i = 0; i < kC; i++)
+        if (si == i) { s_ci = cgi[i]; s_cj = cgj[i]; s_cv = cval[i]; s_cl = wave + kW * i; }
+    const int s_lx = s_cl % kTX, s_ly = s_cl / kTX;
+    double acc[kC][kJ];
+#pragma unroll
+    for (int i = 0; i < kC; i++)
+#pragma unroll
+        for (int j = 0; j < kJ; j++) acc[i][j] = 0.0;
+    double topacc = 0.0;   // G at z1 of this lane's setup cell (cell si)
+    const float zf1 = (float)z1;
+
+    for (uint32_t pb = 0; pb < gc.V; pb += 64) {
+        // ---- windows of views pb .. pb + 63, one per lane of wave 0 ----
+        __syncthreads();   // the previous batch's readers of Win are done
+        if (wave == 0) {
+            GWin wn;
+            wn.qlo = 0; wn.qhi = -1; wn.tlo = 0; wn.W = 2; wn.kc = 0.f; wn.ncg = 1; wn.pad[0] = wn.pad[1] = 0;
+            const uint32_t p = pb + (uint32_t)lane;
+            if (p < gc.V && gviews[p].ok) {
+                const GView g = gviews[p];
+                float qmin = AP_BIG, qmax = -AP_BIG, dmax = 0.f;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const float x = (float)((q & 1) ? X1 : X0), y = (float)((q & 2) ? Y1 : Y0);
+                    const float dx = x - g.sx, dy = y - g.sy;
+                    const float qc = -(g.ax * dy - g.ay * dx) * frcp(g.bx * dy - g.by * dx);
+                    qmin = fminf(qmin, qc);
+                    qmax = fmaxf(qmax, qc);
+                    dmax = fmaxf(dmax, dx * dx + dy * dy);
+                }
+                const int qlo = max(0, (int)floorf(qmin - 1e-3f));
+                const int qhi = min((int)gc.NQ - 1, (int)ceilf(qmax + 1e-3f));
+                if (qmax >= -1.f && qmin <= (float)gc.NQ && qlo <= qhi) {
+                    // horizontal distances: nearest point of the tile box, farthest corner
+                    const float ex = fmaxf(fmaxf((float)X0 - g.sx, g.sx - (float)X1), 0.f);
+                    const float ey = fmaxf(fmaxf((float)Y0 - g.sy, g.sy - (float)Y1), 0.f);
+                    const float hmin = fmaxf(__builtin_amdgcn_sqrtf(ex * ex + ey * ey) * 0.999f, 1e-3f);
+                    const float hmax = __builtin_amdgcn_sqrtf(dmax) * 1.001f;
+                    // source-pixel distance over the window's columns
+                    auto lcol = [&](float q) {
+                        const float x = g.ax + q * g.bx, y = g.ay + q * g.by;
+                        return __builtin_amdgcn_sqrtf(x * x + y * y);
+                    };
+                    const float bb = g.bx * g.bx + g.by * g.by;
+                    const float qv = fminf(fmaxf(-(g.ax * g.bx + g.ay * g.by) * frcp(bb), (float)qlo), (float)qhi);
+                    const float la = lcol((float)qlo), lb = lcol((float)qhi), lv = lcol(qv);
+                    const float lmin = fminf(fminf(la, lb), lv) * 0.999f, lmax = fmaxf(la, lb) * 1.001f;
+                    const float rmin = lmin * frcp(hmax), rmax = lmax * frcp(hmin);
+                    const float kz0 = (float)z0 - g.sz, kz1 = zf1 - g.sz;
+                    const float k00 = kz0 * rmin, k01 = kz0 * rmax, k10 = kz1 * rmin, k11 = kz1 * rmax;
+                    const float ids = frcp(g.dzs);
+                    const float kmn = (fminf(fminf(k00, k01), fminf(k10, k11)) - g.dz0) * ids;
+                    const float kmx = (fmaxf(fmaxf(k00, k01), fmaxf(k10, k11)) - g.dz0) * ids;
+                    const int tlo = (int)fminf(fmaxf(floorf(kmn) - 1.f, 0.f), (float)NT);
+                    const int thi = (int)fminf(fmaxf(floorf(kmx) + 3.f, 0.f), (float)NT);
+                    wn.qlo = qlo;
+                    wn.qhi = qhi;
+                    wn.tlo = tlo;
+                    wn.W = thi - tlo + 2;   // P[tlo .. thi + 1]
+                    wn.kc = 1.f - (float)tlo - g.dz0 / g.dzs;
+                    wn.ncg = min(kGMaxCols, kGCap / ((wn.W + 31) & ~31));
+                }
+            }
+            Win[lane] = wn;
+        }
+        __syncthreads();
+        const uint32_t pe = min(gc.V, pb + 64);
+        for (uint32_t p = pb; p < pe; p++) {
+            const GWin wn = Win[p - pb];
+            if (wn.qlo > wn.qhi) continue;   // uniform: the view misses the tile (or is not handled here)
+            const GView g = gviews[p];
+            const int W = wn.W, tlo = wn.tlo;
+            const int Wp = (W + 31) & ~31;   // column stride of the window, whole 256-byte chunks
+            const uint64_t colbase = (uint64_t)p * gc.NQ;
+            double kzd[kJ];
+#pragma unroll
+            for (int jj = 0; jj < kJ; jj++) kzd[jj] = (double)(z0 + lane + 64 * jj) - (double)g.sz;
+            const double kz1d = (double)z1 - (double)g.sz;
+            const float kzspan = fmaxf(fabsf((float)z0 - g.sz), fabsf(zf1 - g.sz));
+            const float Wm2 = (float)(W - 2);
+            const float msz = -g.sz;
+            for (int g0 = wn.qlo; g0 <= wn.qhi; g0 += wn.ncg) {
+                const int g1 = min(wn.qhi, g0 + wn.ncg - 1);
+                const int ng = g1 - g0 + 1;
+                __syncthreads();   // the previous group's readers are done
+                // window rows tlo .. tlo + Wp - 1 of each column, whole 256-byte
+                // chunks straight into LDS (padded rows keep them in bounds)
+                {
+                    const int nch = Wp / 32;   // chunks per column
+                    for (int ch = wave; ch < ng * nch; ch += kW) {
+                        const int c = ch / nch, i = ch - c * nch;
+                        const char* src = reinterpret_cast<const char*>(pref + (colbase + g0 + c) * gc.cs + tlo) +
+                                          i * 256 + lane * 4;
+                        __builtin_amdgcn_global_load_lds(src, reinterpret_cast<char*>(Pw) + (c * Wp + i * 32) * 8,
+                                                         4, 0, 0);
+                    }
+                    // the group's column records (80 B each)
+                    for (int ch = wave; ch * 64 < ng * 20; ch += kW) {
+                        const char* src = reinterpret_cast<const char*>(gcols + colbase + g0) + ch * 256 + lane * 4;
+                        __builtin_amdgcn_global_load_lds(src, reinterpret_cast<char*>(Cw) + ch * 256, 4, 0, 0);
+                    }
+                    if (g0 == wn.qlo) {
+                        for (int ch = wave; ch * 32 < Wp; ch += kW) {
+                            const char* src =
+                                reinterpret_cast<const char*>(izt + (uint64_t)p * gc.cs + tlo) + ch * 256 + lane * 4;
+                            __builtin_amdgcn_global_load_lds(src, reinterpret_cast<char*>(Iw) + ch * 256, 4, 0, 0);
+                        }
+                        for (int e = tid; e < (kTX + 1) * (kTY + 1); e += kT) {
+                            const int cx = e % (kTX + 1), cy = e / (kTX + 1);
+                            const float dx = (float)min(X0 + cx, rc.hi[0]) - g.sx;
+                            const float dy = (float)min(Y0 + cy, rc.hi[1]) - g.sy;
+                            Qc[e] = -(g.ax * dy - g.ay * dx) * frcp(g.bx * dy - g.by * dx);
+                        }
+                    }
+                    __builtin_amdgcn_s_waitcnt(0);
+                }
+                __syncthreads();
+                // ---- setup pass: lane (si, ss) takes column qa(cell si) + ss ----
+                float ha = 0.f, dh = 0.f, aa = 0.f, ba = 0.f, ab = 0.f, bbt = 0.f, lq = 0.f, yhd = 0.f, ifh = 0.f;
+                double gm = 0.0;
+                int pb_ = 0, mp = 0, pcol = 0;
+                bool valid = false;
+                {
+                    const float c00 = Qc[s_ly * (kTX + 1) + s_lx], c10 = Qc[s_ly * (kTX + 1) + s_lx + 1];
+                    const float c01 = Qc[(s_ly + 1) * (kTX + 1) + s_lx], c11 = Qc[(s_ly + 1) * (kTX + 1) + s_lx + 1];
+                    const float cmin = fminf(fminf(c00, c10), fminf(c01, c11));
+                    const float cmax = fmaxf(fmaxf(c00, c10), fmaxf(c01, c11));
+                    const int qa = max(g0, (int)floorf(cmin - 1e-3f));
+                    const int qb = min(g1, (int)ceilf(cmax + 1e-3f));
+                    const int q = qa + ss;
+                    if (s_cv && q <= qb) {
+                        const GCol& G = Cw[q - g0];
+                        float h0 = G.hs, h1 = G.hd;
+                        const int blo[2] = {s_ci, s_cj}, bhi[2] = {s_ci + 1, s_cj + 1};
+#pragma unroll
+                        for (int a = 0; a < 2; a++) {
+                            if (G.sg[a] == 0) {
+                                if (G.ifix[a] != blo[a]) h1 = -AP_BIG;
+                                continue;
+                            }
+                            const float ein = (float)(G.sg[a] > 0 ? blo[a] : bhi[a]) - G.P0[a];
+                            const float eout = (float)(G.sg[a] > 0 ? bhi[a] : blo[a]) - G.P0[a];
+                            h0 = fmaxf(h0, tcross(ein, G.r[a], G.c[a]));
+                            h1 = fminf(h1, tcross(eout, G.r[a], G.c[a]));
+                        }
+                        if (h0 < h1) {
+                            valid = true;
+                            ha = h0;
+                            dh = h1 - h0;
+                            aa = G.lods * frcp(h0 + G.hc);
+                            ab = G.lods * frcp(h1 + G.hc);
+                            ba = __builtin_fmaf(-g.sz, aa, wn.kc);
+                            bbt = __builtin_fmaf(-g.sz, ab, wn.kc);
+                            lq = G.l;
+                            gm = G.gam;
+                            pb_ = (q - g0) * Wp * 8;
+                            pcol = q - g0;
+                            yhd = G.yh * dh;
+                            ifh = (float)G.ifixh;
+                            // rays straddling one plane inside the cell lie in
+                            // (min kappa, max kappa], |kappa_a - kappa_b| = |k - sz| |alpha_a - alpha_b|:
+                            // more than one of them needs the general evaluation
+                            mp = kzspan * fabsf(aa - ab) >= 0.99f ? 1 : 0;
+                        }
+                    }
+                }
+                const char* Pb = reinterpret_cast<const char*>(Pw);
+                // G(k) of one (cell, column) pair at boundary kf (kz = kf - sz in FP64), added to a
+                auto gacc = [&](double& a, float kf, double kz, float ha_, float dh_, float aa_, float ba_, float ab_,
+                                float bb_, float l_, double gm_, int pbb) {
+                    const float ka = __builtin_fmaf(kf, aa_, ba_), kb = __builtin_fmaf(kf, ab_, bb_);
+                    const int tf = cvt_flr(__builtin_amdgcn_fmed3f(fminf(ka, kb), 0.f, Wm2));
+                    const double P0 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8);
+                    const double P1 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8 + 8);
+                    const double iz = Iw[tf];
+                    // crossing of plane k by ray tf, along h - hc, relative to the cell entry
+                    const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
+                    const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
+                    const float F = iz > 0.0 ? Fp : dh_ - Fp;
+                    a = fma((double)dh_, P0, a);
+                    a = fma((double)F, P1 - P0, a);
+                };
+                // the same with every ray that may straddle plane k
+                auto gacc_multi = [&](double& a, float kf, double kz, float ha_, float dh_, float aa_, float ba_,
+                                      float ab_, float bb_, float l_, double gm_, int pbb) {
+                    const float ka = __builtin_fmaf(kf, aa_, ba_), kb = __builtin_fmaf(kf, ab_, bb_);
+                    const int tf = cvt_flr(__builtin_amdgcn_fmed3f(fminf(ka, kb), 0.f, Wm2));
+                    const int te = cvt_flr(__builtin_amdgcn_fmed3f(fmaxf(ka, kb), (float)tf, Wm2));
+                    a = fma((double)dh_, *reinterpret_cast<const double*>(Pb + pbb + tf * 8), a);
+                    for (int t = tf; t <= te; t++) {
+                        const double iz = Iw[t];
+                        const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
+                        const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
+                        const float F = iz > 0.0 ? Fp : dh_ - Fp;
+                        const double P0 = *reinterpret_cast<const double*>(Pb + pbb + t * 8);
+                        const double P1 = *reinterpret_cast<const double*>(Pb + pbb + t * 8 + 8);
+                        a = fma((double)F, P1 - P0, a);
+                    }
+                };
+                {
+                    // flags: bit 24 several rays may straddle a plane, bit 25 the
+                    // column has a ray lying in a z plane, bits 26-29 its column in
+                    // the group (Cw); bits 0-23 the window's byte offset
+                    float* R = Prec[wave][lane];
+                    const uint64_t gbits = (uint64_t)__double_as_longlong(gm);
+                    const int flags = pb_ | (mp << 24) | ((yhd != 0.f ? 1 : 0) << 25) | (pcol << 26);
+                    reinterpret_cast<float4*>(R)[0] = make_float4(ha, dh, aa, ab);
+                    reinterpret_cast<float4*>(R)[1] = make_float4(
+                        lq, __int_as_float(flags), __uint_as_float((uint32_t)gbits), __uint_as_float((uint32_t)(gbits >> 32)));
+                }
+                if (valid) {
+                    double gt = 0.0;
+                    if (mp)
+                        gacc_multi(gt, zf1, kz1d, ha, dh, aa, ba, ab, bbt, lq, gm, pb_);
+                    else
+                        gacc(gt, zf1, kz1d, ha, dh, aa, ba, ab, bbt, lq, gm, pb_);
+                    if (yhd != 0.f && zf1 > ifh) gt += (double)yhd;
+                    topacc += gt;
+                }
+                // ---- main pass: every lane, every pair of cell i ----
+#pragma unroll
+                for (int i = 0; i < kC; i++) {
+                    uint64_t m = __ballot(valid && si == i);
+                    while (m) {
+                        const int src = __builtin_ctzll(m);
+                        m &= m - 1;
+                        // the pair's record, read by every lane at one address (a broadcast)
+                        const float4* R = reinterpret_cast<const float4*>(Prec[wave][src]);
+                        const float4 r0 = R[0], r1 = R[1];
+                        const float rha = r0.x, rdh = r0.y, raa = r0.z, rab = r0.w;
+                        const float rba = __builtin_fmaf(msz, raa, wn.kc), rbb = __builtin_fmaf(msz, rab, wn.kc);
+                        const float rl = r1.x;
+                        const int rfl = __float_as_int(r1.y);
+                        const int rpb = rfl & 0xFFFFFF;
+                        const double rgm = __longlong_as_double(
+                            (long long)(((uint64_t)__float_as_uint(r1.w) << 32) | __float_as_uint(r1.z)));
+                        if (rfl & (1 << 24)) {
+                            for (int jj = 0; jj < kJ; jj++)
+                                gacc_multi(acc[i][jj], (float)(z0 + lane + 64 * jj), kzd[jj], rha, rdh, raa, rba, rab,
+                                           rbb, rl, rgm, rpb);
+                        } else {
+#pragma unroll
+                            for (int jj = 0; jj < kJ; jj++)
+                                gacc(acc[i][jj], (float)(z0 + lane + 64 * jj), kzd[jj], rha, rdh, raa, rba, rab, rbb,
+                                     rl, rgm, rpb);
+                        }
+                        if (rfl & (1 << 25)) {
+                            // rays lying in a z plane: their voxel row is below the plane
+                            const GCol& G = Cw[(rfl >> 26) & 15];
+                            const double ryh = (double)(G.yh * rdh);
+                            const float rif = (float)G.ifixh;
+#pragma unroll
+                            for (int jj = 0; jj < kJ; jj++)
+                                if ((float)(z0 + lane + 64 * jj) > rif) acc[i][jj] += ryh;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // ---- G at z1 per cell: sum over the lanes of the cell's setup slots ----
+    for (int o = kC; o < 64; o <<= 1) topacc += __shfl_xor(topacc, o, 64);
+    // ---- voxel values: differences of neighbouring boundaries ----
+#pragma unroll
+    for (int i = 0; i < kC; i++) {
+        const double top = __shfl(topacc, i, 64);
+        if (!cval[i]) continue;
+#pragma unroll
+        for (int jj = 0; jj < kJ; jj++) {
+            const int k = z0 + lane + 64 * jj;
+            double nxt = __shfl_down(acc[i][jj], 1, 64);
+            if (jj + 1 < kJ) {
+                const double n0 = __shfl(acc[i][jj + 1], 0, 64);
+                if (lane == 63) nxt = n0;
+            }
+            if (k + 1 == z1) nxt = top;
+            if (k < z1) {
+                const float v = (float)((nxt - acc[i][jj]) * scale);
+                const int64_t ov = (int64_t)cgi[i] * vsx + (int64_t)cgj[i] * vsy + (int64_t)k * vsz;
+                const float mv = mult ? to_f32(mult[(int64_t)cgi[i] * msx + (int64_t)cgj[i] * msy + (int64_t)k * msz])
+                                      : 1.f;
+                vol[ov] = from_f32<Tv>(__builtin_fmaf(mv, v, to_f32(vol[ov])));
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+uint64_t gather_workspace_bytes(uint32_t V, uint32_t nu, uint32_t nv, uint32_t ssu, uint32_t ssv) {
+    if (V == 0 || nu == 0 || nv == 0) return 0;
+    return (uint64_t)g_layout(V, nu * ssu, nv * ssv).total;
+}
+
+bool gather_supported(uint32_t nv, uint32_t ssv) {
+    return (uint64_t)nv * ssv + 34 <= (uint64_t)kGRowMax;
+}
+
+__device__ bool gather_view_ok(const RayCfg& rc, const ap_proj_spec* __restrict__ specs, uint32_t p, uint32_t ssu,
+                               uint32_t ssv, uint32_t nv) {
+    GView gv;
+    return gather_view(rc, specs, p, ssu, ssv, nv * ssv, gv);
+}
+
+namespace {
+
+constexpr int kGW = 8, kGCv = 4, kGJ = 4, kGTX = 8;
+
+}  // namespace
+
+int32_t launch_gather_adjoint(const RayCfg& rc, uint32_t V, uint32_t nu, uint32_t nv, int64_t s0, int64_t s1,
+                              int64_t s2, uint32_t ssu, uint32_t ssv, double scale, const ap_proj_spec* specs,
+                              const ap_array3* vol_desc, void* vol, int32_t mdt, const int64_t* ms,
+                              const void* mult, int32_t sino_dtype, const void* sino, void* workspace,
+                              uint64_t ws_bytes, hipStream_t s, const uint32_t** left) {
+    if (!gather_supported(nv, ssv)) return AP_ERR_UNSUPPORTED;
+    GCfg gc;
+    gc.V = V;
+    gc.nu = nu;
+    gc.nv = nv;
+    gc.ssu = ssu;
+    gc.ssv = ssv;
+    gc.NQ = nu * ssu;
+    gc.NT = nv * ssv;
+    gc.s0 = s0;
+    gc.s1 = s1;
+    gc.s2 = s2;
+    const GLayout L = g_layout(V, gc.NQ, gc.NT);
+    gc.cs = L.cs;
+    if (!workspace || ws_bytes < L.total || ((uintptr_t)workspace & 255)) return AP_ERR_UNSUPPORTED;
+    char* ws = (char*)workspace;
+    GView* gviews = (GView*)(ws + L.views);
+    GCol* gcols = (GCol*)(ws + L.cols);
+    double* pref = (double*)(ws + L.pref);
+    double* izt = (double*)(ws + L.iz);
+    const uint64_t nwave = (uint64_t)V * gc.NQ;
+    if ((nwave + 3) / 4 >= (1ull << 31)) return AP_ERR_UNSUPPORTED;
+    int32_t st = dispatch_dtype(sino_dtype, [&](auto tp) -> int32_t {
+        using Tp = decltype(tp);
+        if constexpr (std::is_same<Tp, double>::value || std::is_same<Tp, bf16_t>::value) {
+            return ap_fail(AP_ERR_UNSUPPORTED, "sinogram dtype must be F32 or F16");
+        } else {
+            hipLaunchKernelGGL(k_gather_prep<Tp>, dim3((uint32_t)((nwave + 3) / 4)), dim3(256), 0, s, rc, gc, specs,
+                               (const Tp*)sino, gviews, gcols, pref, izt);
+            return ap_check_launch("gather_prep");
+        }
+    });
+    if (st) return st;
+    if (left) {
+        uint32_t* cnt = (uint32_t*)(ws + L.count);
+        hipLaunchKernelGGL(k_gather_count, dim3(1), dim3(256), 0, s, gviews, V, cnt);
+        if ((st = ap_check_launch("gather_count"))) return st;
+        *left = cnt;
+    }
+    constexpr int kTY = kGW * kGCv / kGTX;
+    GTile tl;
+    tl.ntx = (uint32_t)(rc.hi[0] - rc.lo[0] + kGTX - 1) / kGTX;
+    tl.nty = (uint32_t)(rc.hi[1] - rc.lo[1] + kTY - 1) / kTY;
+    tl.ntz = (uint32_t)(rc.hi[2] - rc.lo[2] + 64 * kGJ - 1) / (64 * kGJ);
+    const uint64_t total = (uint64_t)tl.ntx * tl.nty * tl.ntz;
+    if (total >= (1ull << 31)) return ap_fail(AP_ERR_INVALID, "too many tiles");
+    tl.total = (uint32_t)total;
+    tl.per_xcd = (tl.total + kXcdG - 1) / kXcdG;
+    dim3 grid(tl.per_xcd * kXcdG);
+    return dispatch_dtype(vol_desc->dtype, [&](auto tv) -> int32_t {
+        using Tv = decltype(tv);
+        return dispatch_dtype(mdt, [&](auto tm) -> int32_t {
+            using Tm = decltype(tm);
+            if constexpr (std::is_same<Tv, double>::value || std::is_same<Tm, double>::value) {
+                return ap_fail(AP_ERR_UNSUPPORTED, "unsupported dtype combination");
+            } else {
+                hipLaunchKernelGGL((k_gather_adjoint<Tv, Tm, kGW, kGCv, kGJ, kGTX>), grid, dim3(kGW * 64), 0, s, rc,
+                                   gc, tl, gviews, gcols, pref, izt, scale, (Tv*)vol, vol_desc->strides[0],
+                                   vol_desc->strides[1], vol_desc->strides[2], (const Tm*)mult, ms[0], ms[1], ms[2]);
+                return ap_check_launch("gather_adjoint");
+            }
+        });
+    });
+}

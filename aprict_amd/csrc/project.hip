@@ -19,7 +19,7 @@
 // finds max|y| for the scale); the brick is written to HBM once.  No global
 // atomics, one write per voxel, and the result does not depend on the order
 // of the adds.
-#include "siddon.h"
+#include "gather.h"
 
 namespace {
 
@@ -122,6 +122,16 @@ struct BrickView {
     int j0, j1, k0, k1;   // detector footprint (inclusive), empty when j1 < j0
 };
 
+// x rounded to the nearest integer (ties to even), as a 64-bit two's
+// complement value, for |x| < 2^51: adding 1.5 * 2^52 in FP64 leaves
+// 2^51 + rint(x) in the mantissa field, so the bit pattern minus that of
+// 1.5 * 2^52 is rint(x).  (The low word of the constant is 0: one 32-bit
+// subtraction on the high word.)
+__device__ __forceinline__ unsigned long long fx_term64(float x) {
+    const double d = (double)x + 6755399441055744.0;
+    return (unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull;
+}
+
 // Pixel coordinates of the central projection of X (voxel units rel. centre):
 // X = S + lam (ju U + kv V + Q00 - S) solved with the dual basis of (U, V).
 __device__ __forceinline__ bool project_point(const ViewRays& vd, const float X[3], float& ju, float& kv) {
@@ -162,15 +172,21 @@ __global__ __launch_bounds__(kAdjThreads, 4) void k_backproject_matched(
     RayCfg rc, SinoCfg sc, float scale, uint32_t ssu, uint32_t ssv,
     const ap_proj_spec* __restrict__ specs, Tv* __restrict__ vol, IdxT vsx, IdxT vsy, IdxT vsz,
     const Tm* __restrict__ mult, IdxT msx, IdxT msy, IdxT msz, const Tp* __restrict__ sino,
-    uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t per_xcd, int skip_upright, float unit) {
-    // 64-bit fixed point: terms rint(w * y * scale * S) with S = unit /
-    // max|y * scale| over the brick's footprints (|w| <= sqrt(3) voxels, so a
-    // term stays below 2 * unit, and adjoint_fixed_point_unit sizes the unit
-    // so that no sum overflows); integer sums do not depend on the order of
-    // the adds, so the result is deterministic.
+    uint32_t nbx, uint32_t nby, uint32_t nbz, uint32_t per_xcd, int skip_gather, const uint32_t* gather_left) {
+    // 64-bit fixed point: terms rint(w * y * scale * S), each an exact
+    // rounding of the FP32 product to an integer.  T bounds the terms that
+    // can reach one voxel (the sub-rays through the brick's footprints over
+    // all views: a ray crosses a voxel once) and |w| <= sqrt(3) < 2 voxels,
+    // so S = 2^61 / (T max|y * scale|) keeps every sum below 2^62 while the
+    // terms keep their FP32 precision relative to their own size.  Integer
+    // sums do not depend on the order of the adds: the result is
+    // deterministic.
     __shared__ unsigned long long acc[kBX * kBY * kBZ];
     __shared__ BrickView bv[kViewBatchAdj];
     __shared__ float wred[kAdjThreads / 64];
+    __shared__ double tred[kAdjThreads / 64];
+    // every view went to the gather kernel: nothing left here (uniform)
+    if (gather_left && *gather_left == 0u) return;
     const uint32_t L = blockIdx.x;
     const uint32_t logical = (L % kXcd) * per_xcd + L / kXcd;
     const uint32_t nbr = nbx * nby * nbz;
@@ -185,7 +201,8 @@ __global__ __launch_bounds__(kAdjThreads, 4) void k_backproject_matched(
     const float half_u = 0.5f, half_v = 0.5f;
     // footprints of views [p0, p0 + nb) into bv (one thread per view)
     auto fill_views = [&](uint32_t p0, uint32_t nb) {
-        if (tid < (int)nb && skip_upright && view_is_upright(specs, p0 + tid)) {
+        GView gv;
+        if (tid < (int)nb && skip_gather && gather_view(rc, specs, p0 + tid, ssu, ssv, sc.nv * ssv, gv)) {
             bv[tid].j0 = 0; bv[tid].j1 = -1; bv[tid].k0 = 0; bv[tid].k1 = -1;
         } else if (tid < (int)nb) {
             BrickView& B = bv[tid];
@@ -216,11 +233,14 @@ __global__ __launch_bounds__(kAdjThreads, 4) void k_backproject_matched(
     };
     // pass 1: largest |y| over the footprints sets the fixed-point scale
     float ymax = 0.f;
+    double nterm = 0.0;   // footprint sub-rays of this thread's views
     for (uint32_t p0 = 0; p0 < sc.V; p0 += kViewBatchAdj) {
         const uint32_t nb = min((uint32_t)kViewBatchAdj, sc.V - p0);
         __syncthreads();
         fill_views(p0, nb);
         __syncthreads();
+        if (tid < (int)nb && bv[tid].j1 >= bv[tid].j0 && bv[tid].k1 >= bv[tid].k0)
+            nterm += (double)(bv[tid].j1 - bv[tid].j0 + 1) * (double)(bv[tid].k1 - bv[tid].k0 + 1);
         for (uint32_t q = 0; q < nb; q++) {
             const BrickView& B = bv[q];
             if (B.j1 < B.j0 || B.k1 < B.k0) continue;
@@ -235,11 +255,22 @@ __global__ __launch_bounds__(kAdjThreads, 4) void k_backproject_matched(
             }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) ymax = fmaxf(ymax, __shfl_xor(ymax, o, 64));
-    if ((tid & 63) == 0) wred[tid >> 6] = ymax;
+    for (int o = 32; o > 0; o >>= 1) {
+        ymax = fmaxf(ymax, __shfl_xor(ymax, o, 64));
+        nterm += __shfl_xor(nterm, o, 64);
+    }
+    if ((tid & 63) == 0) {
+        wred[tid >> 6] = ymax;
+        tred[tid >> 6] = nterm;
+    }
     __syncthreads();
     ymax = 0.f;
-    for (int w = 0; w < kAdjThreads / 64; w++) ymax = fmaxf(ymax, wred[w]);
+    nterm = 0.0;
+    for (int w = 0; w < kAdjThreads / 64; w++) {
+        ymax = fmaxf(ymax, wred[w]);
+        nterm += tred[w];
+    }
+    nterm *= (double)ssu * (double)ssv;
     const float amax = ymax * fabsf(scale);
     if (amax == 0.f) return;   // block-uniform: nothing to add
     if (!(amax < __builtin_inff())) {
@@ -252,7 +283,9 @@ __global__ __launch_bounds__(kAdjThreads, 4) void k_backproject_matched(
         }
         return;
     }
-    const float S = unit / amax;
+    // terms below 2^50 (the exact FP64 rounding of fx_term64), sums below 2^62
+    const double S = fmin(2305843009213693952.0 / fmax(nterm, 1.0), 1125899906842624.0) / (double)amax;
+    const double scaleS = (double)scale * S;
     for (uint32_t p0 = 0; p0 < sc.V; p0 += kViewBatchAdj) {
         const uint32_t nb = min((uint32_t)kViewBatchAdj, sc.V - p0);
         __syncthreads();
@@ -268,7 +301,7 @@ __global__ __launch_bounds__(kAdjThreads, 4) void k_backproject_matched(
                 const int j = B.j0 + pi / nk, k = B.k0 + pi % nk;
                 const float y = to_f32(sino[(int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2]);
                 if (y == 0.f) continue;
-                const float ys = y * scale * S;
+                const float ys = (float)((double)y * scaleS);
                 for (uint32_t su = 0; su < ssu; su++)
                     for (uint32_t sv = 0; sv < ssv; sv++) {
                         Ray R;
@@ -286,7 +319,7 @@ __global__ __launch_bounds__(kAdjThreads, 4) void k_backproject_matched(
                             float w;
                             int o;
                             const bool more = W.step(w, o);
-                            atomicAdd(&acc[o], (unsigned long long)(long long)(int)__builtin_rintf(w * ys));
+                            atomicAdd(&acc[o], fx_term64(w * ys));
                             if (!more) break;
                         }
                     }
@@ -313,12 +346,6 @@ int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32
                                int64_t s2, float mba, uint32_t ssu, uint32_t ssv, const ap_proj_spec* specs,
                                const ap_array3* vol_desc, const void* vol, int32_t sino_dtype, void* sino,
                                const void* b, const int64_t* bs, bool idx64, hipStream_t s);
-int32_t launch_backproject_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32_t nv, int64_t s0,
-                                   int64_t s1, int64_t s2, float scale, uint32_t ssu, uint32_t ssv,
-                                   const ap_proj_spec* specs, const ap_array3* vol_desc, void* vol,
-                                   int32_t mdt, const int64_t* ms, const void* mult, int32_t sino_dtype,
-                                   const void* sino, bool idx64, hipStream_t s);
-
 namespace {
 
 int32_t check_sino(const ap_array3* sd, const void* sino, uint32_t num_projs, SinoCfg& sc,
@@ -454,7 +481,8 @@ extern "C" int32_t ap_backproject_matched(const ap_config_cb_project* cfg,
                                           const ap_array3* vol_desc, void* vol,
                                           const ap_array3* mult_desc, const void* multiplier,
                                           const ap_array3* sino_desc, const void* sino,
-                                          float alpha, void* stream) {
+                                          float alpha, void* workspace, uint64_t workspace_bytes,
+                                          void* stream) {
     AP_REQUIRE(specs, "specs is NULL");
     int32_t st = ap_check_array3(vol_desc, vol, "volume", true, true, false);
     if (st) return st;
@@ -483,12 +511,35 @@ extern "C" int32_t ap_backproject_matched(const ap_config_cb_project* cfg,
     hipStream_t s = (hipStream_t)stream;
     const int32_t mdt = multiplier ? mult_desc->dtype : AP_F32;
     const int64_t* ms = multiplier ? mult_desc->strides : vol_desc->strides;
-    st = launch_backproject_columns(rc, sc.V, sc.nu, sc.nv, sc.s0, sc.s1, sc.s2, scale, cfg->ssu, cfg->ssv,
-                                    specs, vol_desc, vol, mdt, ms, multiplier, sino_desc->dtype, sino, idx64, s);
-    if (st != AP_OK && st != AP_ERR_UNSUPPORTED) return st;
-    const int skip = st == AP_OK ? 1 : 0;
-    const float unit = adjoint_fixed_point_unit((double)sc.V * sc.nu * sc.nv * cfg->ssu * cfg->ssv);
-    AP_REQUIRE(unit > 0.f, "too many views x rays per view for the adjoint");
+    // upright views go to the gather kernel (one pass when the workspace
+    // holds every view, else view chunks that fit); the general kernel takes
+    // the others
+    int skip = 0;
+    const uint32_t* left = nullptr;
+    const uint64_t need = gather_workspace_bytes(sc.V, sc.nu, sc.nv, cfg->ssu, cfg->ssv);
+    if (workspace && gather_supported(sc.nv, cfg->ssv) && need > 0) {
+        uint32_t vc = sc.V;
+        if (workspace_bytes < need) {
+            vc = (uint32_t)((double)workspace_bytes / ((double)need / sc.V));
+            while (vc > 0 && gather_workspace_bytes(vc, sc.nu, sc.nv, cfg->ssu, cfg->ssv) > workspace_bytes) vc--;
+        }
+        if (vc > 0) {
+            const double dscale = (double)alpha * cfg->multiply_before_adding / ((double)cfg->ssu * cfg->ssv) *
+                                  chunk->voxwid;
+            const int64_t esz = dtype_size(sino_desc->dtype);
+            for (uint32_t p0 = 0; p0 < sc.V; p0 += vc) {
+                const uint32_t n = min(vc, sc.V - p0);
+                st = launch_gather_adjoint(rc, n, sc.nu, sc.nv, sc.s0, sc.s1, sc.s2, cfg->ssu, cfg->ssv, dscale,
+                                           specs + p0, vol_desc, vol, mdt, ms, multiplier, sino_desc->dtype,
+                                           (const char*)sino + (int64_t)p0 * sc.s0 * esz, workspace,
+                                           workspace_bytes, s, vc == sc.V ? &left : nullptr);
+                if (st != AP_OK && st != AP_ERR_UNSUPPORTED) return st;
+                if (st == AP_ERR_UNSUPPORTED) break;
+            }
+            skip = st == AP_OK ? 1 : 0;
+            if (!skip) left = nullptr;
+        }
+    }
     return dispatch_dtype(vol_desc->dtype, [&](auto tv) -> int32_t {
         using Tv = decltype(tv);
         return dispatch_dtype(sino_desc->dtype, [&](auto tp) -> int32_t {
@@ -504,18 +555,25 @@ extern "C" int32_t ap_backproject_matched(const ap_config_cb_project* cfg,
                                            0, s, rc, sc, scale, cfg->ssu, cfg->ssv, specs, (Tv*)vol,
                                            vol_desc->strides[0], vol_desc->strides[1], vol_desc->strides[2],
                                            (const Tm*)multiplier, ms[0], ms[1], ms[2], (const Tp*)sino, nbx,
-                                           nby, nbz, per_xcd, skip, unit);
+                                           nby, nbz, per_xcd, skip, left);
                     } else {
                         hipLaunchKernelGGL((k_backproject_matched<Tv, Tp, Tm, int32_t>), grid, dim3(kAdjThreads),
                                            0, s, rc, sc, scale, cfg->ssu, cfg->ssv, specs, (Tv*)vol,
                                            (int32_t)vol_desc->strides[0], (int32_t)vol_desc->strides[1],
                                            (int32_t)vol_desc->strides[2], (const Tm*)multiplier,
                                            (int32_t)ms[0], (int32_t)ms[1], (int32_t)ms[2], (const Tp*)sino,
-                                           nbx, nby, nbz, per_xcd, skip, unit);
+                                           nbx, nby, nbz, per_xcd, skip, left);
                     }
                     return ap_check_launch("backproject_matched");
                 }
             });
         });
     });
+}
+
+extern "C" uint64_t ap_backproject_matched_workspace_size(const ap_config_cb_project* cfg, uint32_t num_projs,
+                                                          const ap_array3* sino_desc) {
+    if (!cfg || !sino_desc || cfg->ssu < 1 || cfg->ssv < 1) return 0;
+    if (!gather_supported(sino_desc->shape[2], cfg->ssv)) return 0;
+    return gather_workspace_bytes(num_projs, sino_desc->shape[1], sino_desc->shape[2], cfg->ssu, cfg->ssv);
 }

@@ -142,15 +142,16 @@ __global__ __launch_bounds__(kThreads) void k_dot(Shape3 sa, const Ta* a, Shape3
 }
 
 // d/dx_i of sum_j sqrt(|D x_j|^2 + eps^2), D = forward differences with
-// zero flux at the far faces
-__global__ __launch_bounds__(kThreads) void k_tv_grad(Shape3 sx, const float* x, Shape3 sg,
+// zero flux at the far faces; x stored F32, F16 or BF16, arithmetic FP32
+template <typename Tx>
+__global__ __launch_bounds__(kThreads) void k_tv_grad(Shape3 sx, const Tx* x, Shape3 sg,
                                                       float* g, float weight, float eps, uint64_t n) {
     for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n;
          q += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t i, j, k;
-        const int64_t o = offset_of(sx, q, i, j, k);
+        offset_of(sx, q, i, j, k);
         auto X = [&](int64_t ii, int64_t jj, int64_t kk) -> float {
-            return x[ii * sx.s0 + jj * sx.s1 + kk * sx.s2];
+            return to_f32(x[ii * sx.s0 + jj * sx.s1 + kk * sx.s2]);
         };
         auto grad_at = [&](int64_t ii, int64_t jj, int64_t kk, float& a, float& b, float& c) {
             const float v = X(ii, jj, kk);
@@ -173,9 +174,35 @@ __global__ __launch_bounds__(kThreads) void k_tv_grad(Shape3 sx, const float* x,
             grad_at(i, j, k - 1, a, b, c);
             r += c / sqrtf(a * a + b * b + c * c + eps * eps);
         }
-        (void)o;
         const int64_t og = (int64_t)i * sg.s0 + (int64_t)j * sg.s1 + (int64_t)k * sg.s2;
         g[og] += weight * r;
+    }
+}
+
+// sum_j sqrt(|D x_j|^2 + eps^2), accumulated in FP64 into *out
+template <typename Tx>
+__global__ __launch_bounds__(kThreads) void k_tv_value(Shape3 sx, const Tx* x, float eps, uint64_t n,
+                                                       double* out) {
+    double acc = 0.0;
+    for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n;
+         q += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t i, j, k;
+        const int64_t o = offset_of(sx, q, i, j, k);
+        const float v = to_f32(x[o]);
+        const float a = i + 1 < sx.n0 ? to_f32(x[o + sx.s0]) - v : 0.f;
+        const float b = j + 1 < sx.n1 ? to_f32(x[o + sx.s1]) - v : 0.f;
+        const float c = k + 1 < sx.n2 ? to_f32(x[o + sx.s2]) - v : 0.f;
+        acc += (double)sqrtf(a * a + b * b + c * c + eps * eps);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    __shared__ double part[kThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kThreads / 64; w++) t += part[w];
+        atomicAdd(out, t);
     }
 }
 
@@ -322,16 +349,46 @@ int32_t ap_array_dot(const ap_array3* a_desc, const void* a, const ap_array3* b_
 
 int32_t ap_tv_gradient(const ap_array3* x_desc, const void* x, const ap_array3* g_desc, void* grad,
                        float weight, float eps, void* stream) {
-    int32_t st = ap_check_array3(x_desc, x, "x", false, false, false);
+    int32_t st = ap_check_array3(x_desc, x, "x", true, true, false);
     if (st) return st;
     if ((st = ap_check_array3(g_desc, grad, "grad", false, false, false))) return st;
     if ((st = same_shape(x_desc, g_desc))) return st;
     AP_REQUIRE(eps > 0.f, "eps must be positive");
     const uint64_t n = (uint64_t)array3_numel(x_desc);
     if (n == 0) return AP_OK;
-    hipLaunchKernelGGL(k_tv_grad, dim3(grid_for(n)), dim3(kThreads), 0, (hipStream_t)stream,
-                       shape_of(x_desc), (const float*)x, shape_of(g_desc), (float*)grad, weight, eps, n);
-    return ap_check_launch("tv_gradient");
+    return dispatch_dtype(x_desc->dtype, [&](auto tx) -> int32_t {
+        using Tx = decltype(tx);
+        if constexpr (std::is_same<Tx, double>::value) {
+            return ap_fail(AP_ERR_UNSUPPORTED, "x: float64 not supported here");
+        } else {
+            hipLaunchKernelGGL((k_tv_grad<Tx>), dim3(grid_for(n)), dim3(kThreads), 0, (hipStream_t)stream,
+                               shape_of(x_desc), (const Tx*)x, shape_of(g_desc), (float*)grad, weight, eps, n);
+            return ap_check_launch("tv_gradient");
+        }
+    });
+}
+
+int32_t ap_tv_value(const ap_array3* x_desc, const void* x, float eps, double* out, void* stream) {
+    int32_t st = ap_check_array3(x_desc, x, "x", true, true, false);
+    if (st) return st;
+    AP_REQUIRE(eps > 0.f, "eps must be positive");
+    AP_REQUIRE(out, "out is NULL");
+    const uint64_t n = (uint64_t)array3_numel(x_desc);
+    hipStream_t hs = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, hs, out);
+    if (n == 0) return ap_check_launch("tv_value");
+    uint64_t gr = (n + kThreads * 8 - 1) / (kThreads * 8);
+    if (gr > 8192) gr = 8192;
+    return dispatch_dtype(x_desc->dtype, [&](auto tx) -> int32_t {
+        using Tx = decltype(tx);
+        if constexpr (std::is_same<Tx, double>::value) {
+            return ap_fail(AP_ERR_UNSUPPORTED, "x: float64 not supported here");
+        } else {
+            hipLaunchKernelGGL((k_tv_value<Tx>), dim3((uint32_t)gr), dim3(kThreads), 0, hs, shape_of(x_desc),
+                               (const Tx*)x, eps, n, out);
+            return ap_check_launch("tv_value");
+        }
+    });
 }
 
 int32_t ap_sirt_iterate(const ap_config_cb_backproject* bp_cfg, const ap_proj_spec* specs,

@@ -299,9 +299,10 @@ def backproject_matched_streamed(config: cbfp.ConfigProject, volume: HostVolume,
     cfg = config.contents
 
     def call(lib, ps, data, chunk, vd, base, md, mbase):
+        ws, wsb = _abi.matched_workspace(cfg, data.shape[0], data)
         _abi.check(lib.ap_backproject_matched(
             _abi.struct_ptr(cfg), _abi.ptr(ps.specs_device()), data.shape[0], _abi.struct_ptr(chunk), vd, base,
-            md, mbase, _abi.array3(data), _abi.ptr(data), float(alpha), _abi.stream_handle(data.device)),
+            md, mbase, _abi.array3(data), _abi.ptr(data), float(alpha), ws, wsb, _abi.stream_handle(data.device)),
             "backproject_matched_streamed")
 
     _backproject_streamed(call, volume, projs, brick, views_per_slice, device, summand_multiplier)

@@ -294,15 +294,46 @@ def batched_leg(iters: int = 10) -> dict:
             "prep_s": t_prep, "rel_l2_fp16_vs_fp32": (num / den) ** 0.5}
 
 
-def cpu_baseline(vol_np, pg, n, w, views: int) -> dict:
-    """FP32 OpenMP port of the reference semantics (oracle/liboracle32.so):
-    the reference's snap DDA forward projector and its exact voxel-driven
-    backprojector, on `views` evenly spaced views of the full workload."""
+def host_cpu() -> dict:
+    """The host CPU: model name, logical CPUs, the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count() or 1, "affinity": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(vol_np, pg, n, w, views: int, runs: int = 5) -> dict:
+    """FP32 OpenMP port of the reference semantics (oracle.c built with
+    -march=native on this host): the reference's snap DDA forward projector and
+    its exact voxel-driven backprojector, on `views` evenly spaced views of the
+    full workload; one warm-up run, then the median of `runs`.  Threads: every
+    CPU this process may run on, capped by OMP_NUM_THREADS when that is set
+    (the host's share for this GPU)."""
+    import tempfile
+
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from aprict_amd.cb import flatpanel as cbfp
     from oracle import oracle as orc
 
-    cores = min(os.cpu_count() or 1, 16)
+    cpu = host_cpu()
+    cores = cpu["affinity"]
+    if cpu["omp_num_threads"]:
+        try:
+            cores = max(1, min(cores, int(cpu["omp_num_threads"])))
+        except ValueError:
+            pass
+    build = orc.use_native_port(tempfile.mkdtemp(prefix="oracle_native_"))
     specs_all = pg.to_proj_spec_array()
     idx = np.linspace(0, len(specs_all), views, endpoint=False).astype(int)
     specs = specs_all[idx]
@@ -311,19 +342,64 @@ def cpu_baseline(vol_np, pg, n, w, views: int) -> dict:
                      dtype=__import__("aprict_amd")._abi.vol_tex_chunk_spec)
     nu, nv = int(specs[0]["dspec"]["nu"]), int(specs[0]["dspec"]["nv"])
     cfg = cbfp.ConfigProject().contents
-    t0 = time.perf_counter()
-    sino = orc.project_voxel_refdda(cfg, specs, chunk, vol_np, np.zeros((views, nu, nv), np.float32),
-                                    nthreads=cores)
-    t_fwd = time.perf_counter() - t0
     bp = cbfp.ConfigBackproject(fast_approx=False).contents
-    t0 = time.perf_counter()
-    orc.backproject(bp, specs, chunk, np.zeros_like(vol_np), sino, nthreads=cores, fp32=True)
-    t_bp = time.perf_counter() - t0
+    tf, tb = [], []
+    for _ in range(runs + 1):
+        t0 = time.perf_counter()
+        sino = orc.project_voxel_refdda(cfg, specs, chunk, vol_np, np.zeros((views, nu, nv), np.float32),
+                                        nthreads=cores)
+        tf.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        orc.backproject(bp, specs, chunk, np.zeros_like(vol_np), sino, nthreads=cores, fp32=True)
+        tb.append(time.perf_counter() - t0)
+    t_fwd, t_bp = float(np.median(tf[1:])), float(np.median(tb[1:]))
     gups = 2.0 * n**3 * views / (t_fwd + t_bp) / 1e9
     return {"value": gups, "unit": "GUPS", "cores": cores, "kind": "port",
+            "host": cpu, "build": build,
             "sample": f"{views} of {len(specs_all)} views evenly spaced, full {n}^3 volume and {nu}x{nv} "
                       f"detector; FP32 OpenMP port of the reference's DDA forward projector and exact "
-                      f"voxel-driven backprojector (fwd {t_fwd:.2f} s, bp {t_bp:.2f} s)"}
+                      f"voxel-driven backprojector ({build} build, {cores} threads); 1 warm-up run, median of "
+                      f"{runs}: fwd {t_fwd:.2f} s, bp {t_bp:.2f} s"}
+
+
+def tv_gd_leg(P, truth, iters: int = 5, power_iterations: int = 8) -> dict:
+    """The differentiable workload at 512^3 x 720 x 768^2: b = A x_true +
+    N(0, (0.01 max b)^2) (noise from torch's seeded generator on the device),
+    gradient descent on ||A x - b||^2 + 1e-3 TV_eps(x) from x = 0, with x
+    stored fp32 and bf16.  Reports seconds per iteration (synchronised wall
+    time of `iters` iterations) and the objective before and after."""
+    import torch
+
+    from aprict_amd.iterators import TVGradientDescent, power_iteration
+
+    b = P.A(truth)
+    gen = torch.Generator(device=b.device)
+    gen.manual_seed(4)
+    sigma = 0.01 * float(b.max())
+    b.add_(torch.randn(b.shape, generator=gen, device=b.device, dtype=torch.float32), alpha=sigma)
+    t0 = time.perf_counter()
+    lam = power_iteration(P, power_iterations)
+    torch.cuda.synchronize()
+    t_pow = time.perf_counter() - t0
+    out = {"workload": "gradient descent on ||Ax-b||^2 + 1e-3 TV_eps (eps 1e-3), 512^3, 720 x 768^2, "
+                       "b = A x_true + 1 % Gaussian noise, x0 = 0, step 1/L",
+           "lambda_max": lam, "power_iterations": power_iterations, "s_power_iteration": t_pow}
+    for name, dt in (("f32", torch.float32), ("bf16", torch.bfloat16)):
+        gd = TVGradientDescent(P, b, beta=1e-3, eps=1e-3, dtype=dt, lam=lam)
+        f0 = sum(gd.objective())
+        gd.iterate()   # warm-up (workspace, first launches)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            gd.iterate()
+        torch.cuda.synchronize()
+        per = (time.perf_counter() - t0) / iters
+        d, t = gd.objective()
+        out[name] = {"s_per_iter": per, "objective_start": f0, "objective_end": d + t,
+                     "objective_ratio": (d + t) / f0, "iterations": iters + 1, "tau": gd.tau}
+        del gd
+        torch.cuda.empty_cache()
+    return out
 
 
 def read_traffic(kernel: str):
@@ -362,13 +438,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--volume-dtype", choices=["f32", "bf16"], default="f32")
-    ap.add_argument("--cpu-views", type=int, default=8)
+    ap.add_argument("--cpu-views", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sirt", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-storage", action="store_true")
     ap.add_argument("--no-analytic", action="store_true")
     ap.add_argument("--no-fdk", action="store_true")
+    ap.add_argument("--no-tv", action="store_true")
     ap.add_argument("--analytic-only", action="store_true", help="run only the analytic FORBILD leg")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
@@ -475,6 +552,9 @@ def main() -> int:
     if args.volume_dtype == "f32" and not args.no_storage:
         log("16-bit storage leg")
         out["bf16_storage"] = storage_leg(P, x, y, g, t_a + t_at)
+    if args.volume_dtype == "f32" and not args.no_tv:
+        log("TV-regularised gradient descent 512^3 (fp32 and bf16 storage)")
+        out["tv_gd"] = tv_gd_leg(P, x)
     if not args.no_sirt:
         log("SIRT 256^3 x 360 x 384^2")
         out["sirt"] = sirt_seconds_per_iter()

@@ -228,14 +228,29 @@ int32_t ap_backproject(const ap_config_cb_backproject* cfg,
  * (same Siddon weights, ssu/ssv averaging, shrink factor and
  * multiply_before_adding).  No reference kernel computes this transpose;
  * at ssu=ssv=1 it equals the exact backprojector of CB_backproject.cu:408-477
- * (ConfigBackproject(fast_approx=False) without mask/softening/Z-weights).  */
+ * (ConfigBackproject(fast_approx=False) without mask/softening/Z-weights).
+ *
+ * `workspace` (device memory, 256-byte aligned, may be NULL) holds the
+ * per-column prefix sums of the gather formulation that upright-detector
+ * views take: ap_backproject_matched_workspace_size() bytes let it take all
+ * of them in one pass; less takes them in view chunks that fit (each chunk
+ * adds to vol); NULL or too little for one view leaves every view to the
+ * general (brick) kernel.  Results agree to FP32 rounding in every case.
+ * The workspace is scratch: calls that share one must be ordered.           */
 int32_t ap_backproject_matched(const ap_config_cb_project* cfg,
                                const ap_proj_spec* specs, uint32_t num_projs,
                                const ap_vol_tex_chunk_spec* chunk,
                                const ap_array3* vol_desc, void* vol,
                                const ap_array3* mult_desc, const void* multiplier,
                                const ap_array3* sino_desc, const void* sino,
-                               float alpha, void* stream);
+                               float alpha, void* workspace, uint64_t workspace_bytes,
+                               void* stream);
+/* Workspace bytes for ap_backproject_matched to take every upright view of
+ * a call with this config, view count and sinogram shape in one pass (0 when
+ * the gather formulation does not apply: nv * ssv above 2014).             */
+uint64_t ap_backproject_matched_workspace_size(const ap_config_cb_project* cfg,
+                                               uint32_t num_projs,
+                                               const ap_array3* sino_desc);
 
 /* ---- analytic FORBILD forward projection -------------------------------
  * Replaces the project_CB_FORBILD_shapes launch in cb/flatpanel.py:1042-1109
@@ -352,10 +367,15 @@ int32_t ap_array_dot(const ap_array3* a_desc, const void* a,
                      const ap_array3* b_desc, const void* b,
                      double* out, void* stream);
 /* grad += weight * d/dx sum_i sqrt(|grad_fwd x_i|^2 + eps^2)  (isotropic TV,
- * forward differences, Neumann boundary). x and grad: F32 volumes. */
+ * forward differences, Neumann boundary).  x: F32, F16 or BF16 volume;
+ * grad: F32.  No reference counterpart (aprict has no TV term). */
 int32_t ap_tv_gradient(const ap_array3* x_desc, const void* x,
                        const ap_array3* g_desc, void* grad,
                        float weight, float eps, void* stream);
+/* *out (device double) = sum_i sqrt(|grad_fwd x_i|^2 + eps^2), the value
+ * whose gradient ap_tv_gradient adds, accumulated in FP64 (overwrites *out). */
+int32_t ap_tv_value(const ap_array3* x_desc, const void* x, float eps,
+                    double* out, void* stream);
 
 /* ---- fused SIRT iteration ----------------------------------------------
  * One iterate() of iterators/sirt_vram.py:130-149 with the residual fused

@@ -30,22 +30,40 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
+def use_native_port(outdir: str) -> str:
+    """Build the FP32 port with -march=native for this host into ``outdir``
+    and use it for the fp32 functions from now on.  Returns what was built
+    ("native") or why the portable x86-64-v3 build stays in use."""
+    try:
+        subprocess.run(["make", "-s", "-C", str(HERE), "native", f"NATIVE_OUT={outdir}"], check=True,
+                       capture_output=True, text=True, timeout=300)
+        _LIBS.pop(True, None)
+        _load(True, pathlib.Path(outdir) / "liboracle32_native.so")
+        return "native"
+    except (OSError, subprocess.SubprocessError) as e:
+        return f"x86-64-v3 (native build failed: {e})"
+
+
 def _lib(fp32: bool) -> ctypes.CDLL:
     if fp32 not in _LIBS:
         path = BUILD / ("liboracle32.so" if fp32 else "liboracle64.so")
         if not path.exists():
             build()
-        lib = ctypes.CDLL(str(path))
-        real = ctypes.c_float if fp32 else ctypes.c_double
-        sfx = "_f32" if fp32 else ""
-        getattr(lib, "or_slab_length" + sfx).restype = real
-        getattr(lib, "or_gbc_weight" + sfx).restype = real
-        for name in ("or_project_voxel", "or_backproject", "or_backproject_matched", "or_ramp_filter",
-                     "or_project_forbild", "or_voxelise_forbild", "or_project_voxel_refdda", "or_backproject_gbc",
-                     "or_gbc_weight_integ"):
-            getattr(lib, name + sfx).restype = ctypes.c_int
-        _LIBS[fp32] = lib
+        _load(fp32, path)
     return _LIBS[fp32]
+
+
+def _load(fp32: bool, path: pathlib.Path) -> None:
+    lib = ctypes.CDLL(str(path))
+    real = ctypes.c_float if fp32 else ctypes.c_double
+    sfx = "_f32" if fp32 else ""
+    getattr(lib, "or_slab_length" + sfx).restype = real
+    getattr(lib, "or_gbc_weight" + sfx).restype = real
+    for name in ("or_project_voxel", "or_backproject", "or_backproject_matched", "or_ramp_filter",
+                 "or_project_forbild", "or_voxelise_forbild", "or_project_voxel_refdda", "or_backproject_gbc",
+                 "or_gbc_weight_integ"):
+        getattr(lib, name + sfx).restype = ctypes.c_int
+    _LIBS[fp32] = lib
 
 
 def _fn(name: str, fp32: bool):
@@ -188,3 +206,38 @@ def voxelise_forbild(shapes, chunk, vol, ss: int = 1, *, nthreads: int = 0, fp32
                                           _p(_dims(vol)), _p(vol), ctypes.c_uint32(ss), ctypes.c_int(nthreads))
     _ok(st, "voxelise_forbild")
     return vol
+
+
+# ---------------------------------------------------------------------------
+# smoothed isotropic total variation (numpy, FP64).  aprict has no TV term:
+# the definition is the one the differentiable workload states,
+# TV_eps(x) = sum_i sqrt(|D x_i|^2 + eps^2) with forward differences D that
+# are zero across the far faces.  Pinned in tests/test_oracle_tv.py by
+# closed forms and by central differences of tv_value.
+# ---------------------------------------------------------------------------
+def _tv_diffs(x: np.ndarray):
+    x = np.asarray(x, dtype=np.float64)
+    dx = np.zeros_like(x)
+    dy = np.zeros_like(x)
+    dz = np.zeros_like(x)
+    dx[:-1] = x[1:] - x[:-1]
+    dy[:, :-1] = x[:, 1:] - x[:, :-1]
+    dz[:, :, :-1] = x[:, :, 1:] - x[:, :, :-1]
+    return dx, dy, dz
+
+
+def tv_value(x: np.ndarray, eps: float) -> float:
+    dx, dy, dz = _tv_diffs(x)
+    return float(np.sqrt(dx * dx + dy * dy + dz * dz + eps * eps).sum())
+
+
+def tv_gradient(x: np.ndarray, eps: float) -> np.ndarray:
+    """d tv_value / d x (the adjoint of D applied to D x / |D x|_eps)."""
+    dx, dy, dz = _tv_diffs(x)
+    n = np.sqrt(dx * dx + dy * dy + dz * dz + eps * eps)
+    px, py, pz = dx / n, dy / n, dz / n
+    g = -(px + py + pz)
+    g[1:] += px[:-1]
+    g[:, 1:] += py[:, :-1]
+    g[:, :, 1:] += pz[:, :, :-1]
+    return g

@@ -105,14 +105,16 @@ def test_tilted_adjoint_is_deterministic_and_propagates_nan(device):
 
 @pytest.mark.parametrize("kind", ["upright_one_sign", "upright_signed", "tilted_signed"])
 def test_adjoint_wide_dynamic_range(device, kind):
-    # The adjoints sum 64-bit fixed-point terms whose unit is 2^-29 of the
-    # largest |y| over a brick's footprints.  Here the detector rows below
-    # the centre are four decades dimmer than the rows above, so the voxels
-    # of the lower half (in the same bricks as the upper half for the
-    # upright, column kernel) receive only terms near 1e-4 of the unit's
-    # scale.  A biased rounding of the terms (e.g. truncation) shows as a
-    # systematic relative error of ~1e-5 there; round-to-nearest gives
-    # zero-mean errors near 1e-6.
+    # The detector rows below the centre are four decades dimmer than the rows
+    # above, so the lower half of the volume receives only terms near 1e-4 of
+    # the largest (in the same x-y columns, and for the brick kernel in the
+    # same bricks, as the upper half).  Each voxel's error is measured against
+    # the size of its own terms, (A^T |y|)[i]: an accumulation whose unit is
+    # set by the brightest terms nearby (32-bit fixed point at the scale of a
+    # bright brick) loses about 1e-5 of a dim voxel there, and a biased
+    # rounding shows as a systematic error; FP32 terms summed exactly stay
+    # near 1e-7.  (Relative to |A^T y| itself the error is unbounded where
+    # signed terms cancel, for any FP32 weights.)
     rng = np.random.default_rng(77)
     n, w, nd, V = 32, 0.5, 48, 32
     if kind.startswith("upright"):
@@ -129,22 +131,22 @@ def test_adjoint_wide_dynamic_range(device, kind):
     v = Volume(np.zeros((n, n, n), np.float32), voxwid=w)
     cbfp.backproject_matched(cfg, v, cbfp.ProjectionSet(pg, data=to_gpu(y)), block=True)
     got = v.as_numpy().astype(np.float64)
-    want = orc.backproject_matched(cfg.contents, pg.to_proj_spec_array(), chunk_for((n, n, n), w),
-                                   np.zeros((n, n, n)), y)
-    top = np.abs(want).max()
-    dim = (np.abs(want) > 1e-7 * top) & (np.abs(want) < 1e-3 * top)
+    specs, ch = pg.to_proj_spec_array(), chunk_for((n, n, n), w)
+    want = orc.backproject_matched(cfg.contents, specs, ch, np.zeros((n, n, n)), y)
+    size = orc.backproject_matched(cfg.contents, specs, ch, np.zeros((n, n, n)), np.abs(y))
+    top = size.max()
+    dim = (size > 1e-7 * top) & (size < 1e-3 * top)
     err_all = rel_l2(got, want)
-    err_dim = rel_l2(got[dim], want[dim])
-    bias = float((got[dim] - want[dim]).sum() / np.abs(want[dim]).sum())
-    typical = dim & (np.abs(want) >= 0.1 * np.median(np.abs(want[dim])))
-    per_voxel = float(np.max(np.abs(got[typical] - want[typical]) / np.abs(want[typical])))
-    print(f"adjoint dynamic range ({kind}): {dim.sum()} dim voxels, rel L2 all {err_all:.3e}, dim {err_dim:.3e}, "
-          f"relative bias {bias:.2e}, max per-voxel rel error {per_voxel:.3e}")
+    err_dim = float(np.linalg.norm((got - want)[dim]) / np.linalg.norm(size[dim]))
+    bias = float((got[dim] - want[dim]).sum() / size[dim].sum())
+    per_voxel = float(np.max(np.abs(got[dim] - want[dim]) / size[dim]))
+    print(f"adjoint dynamic range ({kind}): {dim.sum()} dim voxels, rel L2 all {err_all:.3e}, dim (vs A^T|y|) "
+          f"{err_dim:.3e}, relative bias {bias:.2e}, max per-voxel {per_voxel:.3e}")
     assert dim.sum() > n**3 // 8
     assert err_all <= 1e-5
-    assert err_dim <= 5e-6
-    assert abs(bias) <= 1e-6
-    assert per_voxel <= 5e-5
+    assert err_dim <= 1e-6
+    assert abs(bias) <= 2e-7
+    assert per_voxel <= 1e-5
 
 
 def _dot_test(pg, n, w, seed):

@@ -75,7 +75,7 @@ def test_calls_without_views_are_noops(device):
                                 _abi.ptr(sino), None, None, st) == 0
     assert lib.ap_backproject_matched(_abi.struct_ptr(cbfp.ConfigProject().contents), _abi.ptr(specs), 0,
                                       _abi.struct_ptr(chunk), _abi.array3(x), _abi.ptr(x), None, None,
-                                      _abi.array3(sino), _abi.ptr(sino), 1.0, st) == 0
+                                      _abi.array3(sino), _abi.ptr(sino), 1.0, None, 0, st) == 0
     bp = cbfp.ConfigBackproject(fast_approx=True).contents
     assert lib.ap_backproject(_abi.struct_ptr(bp), _abi.ptr(specs), 0, _abi.struct_ptr(chunk), _abi.array3(x),
                               _abi.ptr(x), None, None, _abi.array3(sino), _abi.ptr(sino), None, None, st) == 0
