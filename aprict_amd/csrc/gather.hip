@@ -348,6 +348,7 @@ __global__ __launch_bounds__(kW * 64, 4) void k_gather_adjoint(
             const int W = wn.W, tlo = wn.tlo;
             const int Wp = (W + 31) & ~31;   // column stride of the window, whole 256-byte chunks
             const uint64_t colbase = (uint64_t)p * gc.NQ;
+            // k - sz per slot, exact in FP64 (k an integer below 2^24, sz a float)
             double kzd[kJ];
 #pragma unroll
             for (int jj = 0; jj < kJ; jj++) kzd[jj] = (double)(z0 + lane + 64 * jj) - (double)g.sz;
@@ -363,12 +364,13 @@ __global__ __launch_bounds__(kW * 64, 4) void k_gather_adjoint(
                 // chunks straight into LDS (padded rows keep them in bounds)
                 {
                     const int nch = Wp / 32;   // chunks per column
-                    for (int ch = wave; ch < ng * nch; ch += kW) {
-                        const int c = ch / nch, i = ch - c * nch;
-                        const char* src = reinterpret_cast<const char*>(pref + (colbase + g0 + c) * gc.cs + tlo) +
-                                          i * 256 + lane * 4;
-                        __builtin_amdgcn_global_load_lds(src, reinterpret_cast<char*>(Pw) + (c * Wp + i * 32) * 8,
-                                                         4, 0, 0);
+                    const char* colp = reinterpret_cast<const char*>(pref + (colbase + g0) * gc.cs + tlo) + lane * 4;
+                    const uint64_t cstride = (uint64_t)gc.cs * 8;
+                    for (int c = 0; c < ng; c++) {
+                        const char* src = colp + c * cstride;
+                        char* dst = reinterpret_cast<char*>(Pw) + c * Wp * 8;
+                        for (int i = wave; i < nch; i += kW)
+                            __builtin_amdgcn_global_load_lds(src + i * 256, dst + i * 256, 4, 0, 0);
                     }
                     // the group's column records (80 B each)
                     for (int ch = wave; ch * 64 < ng * 20; ch += kW) {
@@ -511,9 +513,10 @@ __global__ __launch_bounds__(kW * 64, 4) void k_gather_adjoint(
                         const double rgm = __longlong_as_double(
                             (long long)(((uint64_t)__float_as_uint(r1.w) << 32) | __float_as_uint(r1.z)));
                         if (rfl & (1 << 24)) {
-                            for (int jj = 0; jj < kJ; jj++)
-                                gacc_multi(acc[i][jj], (float)(z0 + lane + 64 * jj), kzd[jj], rha, rdh, raa, rba, rab,
-                                           rbb, rl, rgm, rpb);
+                            for (int jj = 0; jj < kJ; jj++) {
+                                const float kf = (float)(z0 + lane + 64 * jj);
+                                gacc_multi(acc[i][jj], kf, kzd[jj], rha, rdh, raa, rba, rab, rbb, rl, rgm, rpb);
+                            }
                         } else {
 #pragma unroll
                             for (int jj = 0; jj < kJ; jj++)
