@@ -154,6 +154,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ap_backproject": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _A3, _P, _P]),
     "ap_backproject_matched": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _F32, _P, ctypes.c_uint64, _P]),
     "ap_backproject_matched_workspace_size": (ctypes.c_uint64, [_P, _U32, _A3]),
+    "ap_backproject_matched_noalloc": (_I32, [_P, _P, _U32, _P, _A3, _P, _A3, _P, _A3, _P, _F32, _P]),
     "ap_project_forbild": (_I32, [_P, _P, _U32, _P, _U32, _A3, _P, _P, _P]),
     "ap_voxelise_forbild": (_I32, [_P, _U32, _P, _A3, _P, _U32, _P]),
     "ap_forbild_bboxes": (_I32, [_P, _U32, _P, _P]),

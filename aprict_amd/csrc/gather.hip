@@ -46,6 +46,8 @@
 // crossing parameters).  A ray lying in a z plane (the forward's horizontal
 // rows, |slope| < 1e-20) belongs to the voxel below the plane, as in the
 // forward; such rays are kept out of P and added separately.
+#include <mutex>
+
 #include "column.h"
 #include "gather.h"
 
@@ -658,4 +660,64 @@ int32_t launch_gather_adjoint(const RayCfg& rc, uint32_t V, uint32_t nu, uint32_
             }
         });
     });
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free entry point
+// ---------------------------------------------------------------------------
+namespace {
+
+// The gather's workspace for callers that cannot allocate: 64 MiB per GPU,
+// held from the first call on that GPU until the process ends.  Host code
+// must not pass the symbol itself to a kernel or a copy (it names a host-side
+// stand-in); the device address of each GPU's copy comes from
+// hipGetSymbolAddress with that GPU current.
+constexpr uint64_t kNoAllocBytes = 64ull << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int32_t scratch_address(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return ap_fail(AP_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return ap_fail(AP_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return ap_fail(AP_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return AP_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t ap_backproject_matched_noalloc(const ap_config_cb_project* cfg, const ap_proj_spec* specs,
+                                                  uint32_t num_projs, const ap_vol_tex_chunk_spec* chunk,
+                                                  const ap_array3* vol_desc, void* vol, const ap_array3* mult_desc,
+                                                  const void* multiplier, const ap_array3* sino_desc,
+                                                  const void* sino, float alpha, void* stream) {
+    AP_REQUIRE(cfg && sino_desc, "NULL config or sinogram descriptor");
+    AP_REQUIRE(cfg->ssu >= 1 && cfg->ssv >= 1, "ssu and ssv must be >= 1");
+    const uint32_t nu = sino_desc->shape[1], nv = sino_desc->shape[2];
+    if (gather_supported(nv, cfg->ssv) && num_projs > 0 && nu > 0 && nv > 0) {
+        const uint64_t one = gather_workspace_bytes(1, nu, nv, cfg->ssu, cfg->ssv);
+        if (one > kNoAllocBytes)
+            return ap_fail(AP_ERR_UNSUPPORTED,
+                           "one view of this detector (%u x %u, ss %u x %u) needs %llu bytes of scratch; the "
+                           "allocation-free adjoint holds %llu (use ap_backproject_matched with a workspace)",
+                           nu, nv, cfg->ssu, cfg->ssv, (unsigned long long)one,
+                           (unsigned long long)kNoAllocBytes);
+    }
+    void* ws = nullptr;
+    int32_t st = scratch_address(&ws);
+    if (st) return st;
+    return ap_backproject_matched(cfg, specs, num_projs, chunk, vol_desc, vol, mult_desc, multiplier, sino_desc,
+                                  sino, alpha, ws, kNoAllocBytes, stream);
 }

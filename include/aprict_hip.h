@@ -251,6 +251,26 @@ int32_t ap_backproject_matched(const ap_config_cb_project* cfg,
 uint64_t ap_backproject_matched_workspace_size(const ap_config_cb_project* cfg,
                                                uint32_t num_projs,
                                                const ap_array3* sino_desc);
+/* ---- matched adjoint that allocates nothing -----------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ap_backproject_matched without the workspace argument.
+ * The gather's prefix sums go to a 64 MiB scratch array that the library
+ * holds on each GPU it runs on, from the first call there until the process
+ * ends, taken in view chunks that fit it (each chunk adds to vol; results
+ * agree with ap_backproject_matched to FP32 rounding).  The call allocates
+ * no device memory.  Returns AP_ERR_UNSUPPORTED, with a message, when one
+ * view of the call needs more than 64 MiB
+ * (ap_backproject_matched_workspace_size(cfg, 1, sino_desc) > 2^26).  All
+ * calls on one GPU share its scratch: calls that overlap in time (on other
+ * streams or host threads) race.                                           */
+int32_t ap_backproject_matched_noalloc(const ap_config_cb_project* cfg,
+                                       const ap_proj_spec* specs, uint32_t num_projs,
+                                       const ap_vol_tex_chunk_spec* chunk,
+                                       const ap_array3* vol_desc, void* vol,
+                                       const ap_array3* mult_desc, const void* multiplier,
+                                       const ap_array3* sino_desc, const void* sino,
+                                       float alpha, void* stream);
 
 /* ---- analytic FORBILD forward projection -------------------------------
  * Replaces the project_CB_FORBILD_shapes launch in cb/flatpanel.py:1042-1109
