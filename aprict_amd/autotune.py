@@ -1,6 +1,6 @@
 """Launch-shape autotuner for the library's tunable kernels.
 
-The library exposes, per tunable operation, a process-wide variant index
+The library exposes, per tunable operation and host thread, a variant index
 (``ap_tune_count/get/set``): the per-ray forward projector's pixels per
 workgroup, the column forward projector's detector rows per workgroup, and the
 voxel-driven backprojector's voxels per workgroup.  All variants of one

@@ -1,6 +1,5 @@
 // abi.hip -- error reporting, struct-layout checks and descriptor validation
 // of the C ABI declared in include/aprict_hip.h.
-#include <atomic>
 #include <stdarg.h>
 #include <stddef.h>
 #include <stdio.h>
@@ -66,12 +65,12 @@ int32_t ap_check_array3(const ap_array3* d, const void* ptr, const char* name,
 
 namespace {
 constexpr int32_t kTuneCount[AP_TUNE_NUM_OPS] = {3, 3, 3};
-std::atomic<int32_t> g_tune[AP_TUNE_NUM_OPS] = {};
+// one selection per host thread: threads that tune or capture different
+// problems at the same time do not see each other's variants
+thread_local int32_t g_tune[AP_TUNE_NUM_OPS] = {};
 }  // namespace
 
-int32_t ap_tune_variant(int32_t op) {
-    return (op >= 0 && op < AP_TUNE_NUM_OPS) ? g_tune[op].load(std::memory_order_relaxed) : 0;
-}
+int32_t ap_tune_variant(int32_t op) { return (op >= 0 && op < AP_TUNE_NUM_OPS) ? g_tune[op] : 0; }
 
 extern "C" {
 
@@ -83,7 +82,7 @@ int32_t ap_tune_set(int32_t op, int32_t variant) {
     if (op < 0 || op >= AP_TUNE_NUM_OPS) return ap_fail(AP_ERR_INVALID, "unknown tuning op %d", op);
     if (variant < 0 || variant >= kTuneCount[op])
         return ap_fail(AP_ERR_INVALID, "tuning op %d has %d variants, got %d", op, kTuneCount[op], variant);
-    g_tune[op].store(variant, std::memory_order_relaxed);
+    g_tune[op] = variant;
     return AP_OK;
 }
 

@@ -43,9 +43,8 @@ struct TileMap {
     uint32_t per_xcd;        // tiles per XCD
 };
 
-__device__ __forceinline__ bool map_tile(const TileMap& tm, uint32_t V, uint32_t& p, uint32_t& jt,
+__device__ __forceinline__ bool map_tile(const TileMap& tm, uint32_t V, uint32_t L, uint32_t& p, uint32_t& jt,
                                          uint32_t& kt) {
-    const uint32_t L = blockIdx.x;
     const uint32_t logical = (L % kXcd) * tm.per_xcd + L / kXcd;
     if (logical >= tm.total) return false;
     kt = logical % tm.nk;
@@ -67,49 +66,68 @@ __global__ __launch_bounds__(kWaveV* kRowsU) void k_project_voxel(
     RayCfg rc, SinoCfg sc, TileMap tm, float mba, uint32_t ssu, uint32_t ssv,
     const ap_proj_spec* __restrict__ specs, const Tv* __restrict__ vol, IdxT vsx, IdxT vsy, IdxT vsz,
     Tp* __restrict__ sino, const Tp* __restrict__ b, int skip_upright) {
-    uint32_t p, jt, kt;
-    if (!map_tile(tm, sc.V, p, jt, kt)) return;
-    if (skip_upright && view_is_upright(specs, p)) return;
-    const uint32_t k = kt * kWaveV + threadIdx.x;
-    const uint32_t j = jt * kRowsU + threadIdx.y;
-    if (k >= sc.nv || j >= sc.nu) return;
-    const ViewRays vr = make_view_rays(rc, specs, p);
-    float tot = 0.f;
-    for (uint32_t su = 0; su < ssu; su++) {
-        float accu = 0.f;
-        for (uint32_t sv = 0; sv < ssv; sv++) {
-            const float ju = (float)j + sub_offset(su, ssu);
-            const float kv = (float)k + sub_offset(sv, ssv);
-            Ray R;
-            if (!ray_setup(vr, rc.lo, rc.hi, ju, kv, R)) continue;
-            ray_start(R, R.t_enter, rc.lo, rc.hi);
-            Walker<IdxT> W;
-            W.init(R, R.t_enter, R.t_exit, vsx, vsy, vsz,
-                   (IdxT)R.idx[0] * vsx + (IdxT)R.idx[1] * vsy + (IdxT)R.idx[2] * vsz);
-            float acc = 0.f;
-            for (int it = 0; it < rc.maxit; it += kBatch) {
-                float w[kBatch];
-                IdxT o[kBatch];
-                bool more = true;
-#pragma unroll
-                for (int q = 0; q < kBatch; q++) more = W.step(w[q], o[q]);
-                float v[kBatch];
-#pragma unroll
-                for (int q = 0; q < kBatch; q++) v[q] = to_f32(vol[o[q]]);
-#pragma unroll
-                for (int q = 0; q < kBatch; q++) acc = __builtin_fmaf(w[q], v[q], acc);
-                if (!more) break;
-            }
-            accu += acc;
+    // A bounded grid strides over the tiles (a multiple of kXcd blocks, so a
+    // block stays in its XCD's range).  The 64 lanes test the block's next 64
+    // tiles at once (in range, view not taken by the column kernel) and the
+    // block walks the ballot of live ones: tiles of skipped views cost a
+    // 64th of one spec load each, not a workgroup launch or a serial load.
+    // The test depends on threadIdx.x only, so every wave sees the same mask.
+    const uint32_t lim = tm.per_xcd * kXcd;
+    for (uint32_t L0 = blockIdx.x; L0 < lim; L0 += 64u * gridDim.x) {
+        bool live;
+        {
+            const uint32_t Lt = L0 + threadIdx.x * gridDim.x;
+            uint32_t p, jt, kt;
+            live = Lt < lim && map_tile(tm, sc.V, Lt, p, jt, kt) && !(skip_upright && view_is_upright(specs, p));
         }
-        tot += ssv > 1 ? accu / (float)ssv : accu;
+        uint64_t mask = __ballot(live);
+        while (mask) {
+            const uint32_t L = L0 + (uint32_t)__builtin_ctzll(mask) * gridDim.x;
+            mask &= mask - 1;
+            uint32_t p, jt, kt;
+            map_tile(tm, sc.V, L, p, jt, kt);
+            const uint32_t k = kt * kWaveV + threadIdx.x;
+            const uint32_t j = jt * kRowsU + threadIdx.y;
+            if (k >= sc.nv || j >= sc.nu) continue;
+            const ViewRays vr = make_view_rays(rc, specs, p);
+            float tot = 0.f;
+            for (uint32_t su = 0; su < ssu; su++) {
+                float accu = 0.f;
+                for (uint32_t sv = 0; sv < ssv; sv++) {
+                    const float ju = (float)j + sub_offset(su, ssu);
+                    const float kv = (float)k + sub_offset(sv, ssv);
+                    Ray R;
+                    if (!ray_setup(vr, rc.lo, rc.hi, ju, kv, R)) continue;
+                    ray_start(R, R.t_enter, rc.lo, rc.hi);
+                    Walker<IdxT> W;
+                    W.init(R, R.t_enter, R.t_exit, vsx, vsy, vsz,
+                           (IdxT)R.idx[0] * vsx + (IdxT)R.idx[1] * vsy + (IdxT)R.idx[2] * vsz);
+                    float acc = 0.f;
+                    for (int it = 0; it < rc.maxit; it += kBatch) {
+                        float w[kBatch];
+                        IdxT o[kBatch];
+                        bool more = true;
+        #pragma unroll
+                        for (int q = 0; q < kBatch; q++) more = W.step(w[q], o[q]);
+                        float v[kBatch];
+        #pragma unroll
+                        for (int q = 0; q < kBatch; q++) v[q] = to_f32(vol[o[q]]);
+        #pragma unroll
+                        for (int q = 0; q < kBatch; q++) acc = __builtin_fmaf(w[q], v[q], acc);
+                        if (!more) break;
+                    }
+                    accu += acc;
+                }
+                tot += ssv > 1 ? accu / (float)ssv : accu;
+            }
+            if (ssu > 1) tot /= (float)ssu;
+            tot *= (float)(1.0 / rc.inv_vw);   // voxel-length units -> physical
+            const int64_t o = (int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2;
+            const float base =
+                to_f32(b ? b[(int64_t)p * sc.b0 + (int64_t)j * sc.b1 + (int64_t)k * sc.b2] : sino[o]);
+            sino[o] = from_f32<Tp>(__builtin_fmaf(mba, tot, base));
+        }
     }
-    if (ssu > 1) tot /= (float)ssu;
-    tot *= (float)(1.0 / rc.inv_vw);   // voxel-length units -> physical
-    const int64_t o = (int64_t)p * sc.s0 + (int64_t)j * sc.s1 + (int64_t)k * sc.s2;
-    const float base =
-        to_f32(b ? b[(int64_t)p * sc.b0 + (int64_t)j * sc.b1 + (int64_t)k * sc.b2] : sino[o]);
-    sino[o] = from_f32<Tp>(__builtin_fmaf(mba, tot, base));
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -433,7 +451,7 @@ extern "C" int32_t ap_project_voxel(const ap_config_cb_project* cfg, const ap_pr
     tm.total = (uint32_t)total;
     tm.per_xcd = (tm.total + kXcd - 1) / kXcd;
     dim3 block(kWaveV, rows_u, 1);
-    dim3 grid(tm.per_xcd * kXcd, 1, 1);
+    dim3 grid(min(tm.per_xcd, 512u) * kXcd, 1, 1);
     hipStream_t s = (hipStream_t)stream;
     const float mba = cfg->multiply_before_adding;
     // upright views go to the column-coherent kernel; the general kernel

@@ -8,70 +8,94 @@
 // taps rounded once from FP64).  The FDK cosine pre-weight of
 // cb/ramp_filter.py:407-436 is fused into the staging of the input.
 //
-// Block: 64 lanes along v (coalesced) x 4 rows; each thread owns 8
-// consecutive outputs j, so the taps a wave reads are uniform (LDS broadcast).
+// Block: 64 lanes along v (coalesced) x 4 rows; a block owns 32 outputs j of
+// one view.  The input rows m are staged through LDS kChunk at a time, each
+// chunk with the taps it needs (so nu is not limited by LDS); the taps a wave
+// reads for one m are uniform (LDS broadcast).
+//  - Ram-Lak (ramp_kernels.py:43-59) is zero at even nonzero offsets, so each
+//    thread owns 8 outputs of one parity (j, j+2, ..., j+14) and visits only
+//    the rows m of the other parity, where every tap is an odd one; the
+//    centre tap (m = j) is added at the end.  Half the multiplies of the
+//    full convolution.
+//  - the other kernels (sinc ramp, Hilbert taps) have no zero taps: each
+//    thread owns 8 consecutive outputs and visits every row.
 #include "ap_common.h"
 
 namespace {
 
 constexpr int kLanes = 64, kRows = 4, kPerThread = 8, kChunk = 128;
 constexpr int kOutPerBlock = kRows * kPerThread;   // 32 outputs j per block
-constexpr int kMaxNu = 4096;
 
+__device__ __forceinline__ double tap_value(int kernel, long d, double p) {
+    const double pi = 3.14159265358979323846;
+    if (kernel == AP_RAMP_SINC) return 8.0 / (1.0 - 4.0 * (double)d * (double)d) / p;
+    if (kernel == AP_RAMP_HILBERT) return d == 0 ? 0.0 : 2.0 / (double)d;   // independent of p
+    if (kernel == AP_RAMP_HILBERT_HALF) return 2.0 / ((double)d + 0.5);
+    if (d == 0) return pi * pi / p;
+    if (d % 2 == 0) return 0.0;
+    return -4.0 / ((double)d * (double)d) / p;
+}
+
+template <bool kRamLak>
 __global__ __launch_bounds__(kLanes* kRows) void k_ramp_rows(
     uint32_t nu, uint32_t nv, const float* __restrict__ in, int64_t i0, int64_t i1, int64_t i2,
     float* __restrict__ out, int64_t o0, int64_t o1, int64_t o2, double p, int kernel, int fdk) {
-    // tap(n) at taps[n + nu - 1]; valid outputs j < nu only read n = j - m in
-    // [-(nu - 1), nu - 1], and the outputs j >= nu of a block's last rows
-    // (discarded) read at most kOutPerBlock entries past 2 nu - 1: padding,
-    // so the loop needs no range check
-    __shared__ float taps[2 * kMaxNu + kOutPerBlock];
+    // taps of offsets d = dlo + e, dlo = J0 - (m0 + kChunk - 1), e < kChunk + kOutPerBlock
+    __shared__ float taps[kChunk + kOutPerBlock];
     __shared__ float xs[kChunk][kLanes];
     const int tid = threadIdx.y * kLanes + threadIdx.x;
     const uint32_t k = blockIdx.x * kLanes + threadIdx.x;
-    const uint32_t jb = blockIdx.y * kOutPerBlock + threadIdx.y * kPerThread;
+    const int J0 = (int)(blockIdx.y * kOutPerBlock);
     const uint32_t view = blockIdx.z;
-    const double pi = 3.14159265358979323846;
-    for (int n = tid; n < (int)(2 * nu - 1); n += kLanes * kRows) {
-        const long d = (long)n - (long)(nu - 1);
-        double t;
-        if (kernel == AP_RAMP_SINC) t = 8.0 / (1.0 - 4.0 * (double)d * (double)d) / p;
-        else if (kernel == AP_RAMP_HILBERT) t = d == 0 ? 0.0 : 2.0 / (double)d;   // independent of p
-        else if (kernel == AP_RAMP_HILBERT_HALF) t = 2.0 / ((double)d + 0.5);
-        else if (d == 0) t = pi * pi / p;
-        else if (d % 2 == 0) t = 0.0;
-        else t = -4.0 / ((double)d * (double)d) / p;
-        taps[n] = (float)t;
-    }
+    // this thread's outputs: j_q = jt + q * js
+    const int ty = (int)threadIdx.y;
+    const int jt = kRamLak ? J0 + 16 * (ty >> 1) + (ty & 1) : J0 + ty * kPerThread;
+    const int js = kRamLak ? 2 : 1;
     float acc[kPerThread];
     for (int q = 0; q < kPerThread; q++) acc[q] = 0.f;
     const float vv = (float)(p * ((double)k - 0.5 * (double)(nv - 1)));
+    auto load_x = [&](uint32_t m) -> float {
+        float x = in[(int64_t)view * i0 + (int64_t)m * i1 + (int64_t)k * i2];
+        if (fdk) {
+            const float uu = (float)(p * ((double)m - 0.5 * (double)(nu - 1)));
+            x *= 1.f / sqrtf(1.f + uu * uu + vv * vv);
+        }
+        return x;
+    };
     for (uint32_t m0 = 0; m0 < nu; m0 += kChunk) {
         const uint32_t nm = min((uint32_t)kChunk, nu - m0);
+        const long dlo = (long)J0 - (long)(m0 + kChunk - 1);
         __syncthreads();
-        for (uint32_t r = threadIdx.y; r < nm; r += kRows) {
-            float x = 0.f;
-            if (k < nv) {
-                x = in[(int64_t)view * i0 + (int64_t)(m0 + r) * i1 + (int64_t)k * i2];
-                if (fdk) {
-                    const float uu = (float)(p * ((double)(m0 + r) - 0.5 * (double)(nu - 1)));
-                    x *= 1.f / sqrtf(1.f + uu * uu + vv * vv);
-                }
-            }
-            xs[r][threadIdx.x] = x;
-        }
+        for (int e = tid; e < kChunk + kOutPerBlock; e += kLanes * kRows) taps[e] = (float)tap_value(kernel, dlo + e, p);
+        for (uint32_t r = threadIdx.y; r < nm; r += kRows) xs[r][threadIdx.x] = k < nv ? load_x(m0 + r) : 0.f;
         __syncthreads();
-        for (uint32_t r = 0; r < nm; r++) {
-            const float x = xs[r][threadIdx.x];
-            const int base = (int)jb - (int)(m0 + r) + (int)nu - 1;
+        // taps[base + q * js] = tap(j_q - m) with base = jt - m - dlo
+        if (kRamLak) {
+            // rows of the other parity than jt: every j_q - m is odd
+            const uint32_t r0 = (uint32_t)((jt + 1 - (int)m0) & 1);
+            for (uint32_t r = r0; r < nm; r += 2) {
+                const float x = xs[r][threadIdx.x];
+                const int base = jt - (int)(m0 + r) - (int)dlo;
 #pragma unroll
-            for (int q = 0; q < kPerThread; q++) acc[q] = __builtin_fmaf(x, taps[base + q], acc[q]);
+                for (int q = 0; q < kPerThread; q++) acc[q] = __builtin_fmaf(x, taps[base + 2 * q], acc[q]);
+            }
+        } else {
+            for (uint32_t r = 0; r < nm; r++) {
+                const float x = xs[r][threadIdx.x];
+                const int base = jt - (int)(m0 + r) - (int)dlo;
+#pragma unroll
+                for (int q = 0; q < kPerThread; q++) acc[q] = __builtin_fmaf(x, taps[base + q], acc[q]);
+            }
         }
     }
     if (k >= nv) return;
+    const float centre = kRamLak ? (float)tap_value(AP_RAMP_RAMLAK, 0, p) : 0.f;
     for (int q = 0; q < kPerThread; q++) {
-        const uint32_t j = jb + q;
-        if (j < nu) out[(int64_t)view * o0 + (int64_t)j * o1 + (int64_t)k * o2] = acc[q];
+        const int j = jt + q * js;
+        if (j >= (int)nu) continue;
+        float a = acc[q];
+        if (kRamLak) a = __builtin_fmaf(load_x((uint32_t)j), centre, a);
+        out[(int64_t)view * o0 + (int64_t)j * o1 + (int64_t)k * o2] = a;
     }
 }
 
@@ -92,12 +116,16 @@ extern "C" int32_t ap_ramp_filter_rows(const ap_array3* in_desc, const void* in,
                "unknown ramp kernel %d", kernel);
     const uint32_t V = in_desc->shape[0], nu = in_desc->shape[1], nv = in_desc->shape[2];
     if (V == 0 || nu == 0 || nv == 0) return AP_OK;
-    AP_REQUIRE(nu <= (uint32_t)kMaxNu, "nu above %d", kMaxNu);
+    AP_REQUIRE(nu < (1u << 30), "nu above 2^30");
     AP_REQUIRE(V <= 65535u, "at most 65535 views per call");
     dim3 grid((nv + kLanes - 1) / kLanes, (nu + kOutPerBlock - 1) / kOutPerBlock, V);
-    hipLaunchKernelGGL(k_ramp_rows, grid, dim3(kLanes, kRows, 1), 0, (hipStream_t)stream, nu, nv,
-                       (const float*)in, in_desc->strides[0], in_desc->strides[1], in_desc->strides[2],
-                       (float*)out, out_desc->strides[0], out_desc->strides[1], out_desc->strides[2],
-                       physpixwid, (int)kernel, (int)fdk_cosine_weight);
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, dim3(kLanes, kRows, 1), 0, (hipStream_t)stream, nu, nv, (const float*)in,
+                           in_desc->strides[0], in_desc->strides[1], in_desc->strides[2], (float*)out,
+                           out_desc->strides[0], out_desc->strides[1], out_desc->strides[2], physpixwid,
+                           (int)kernel, (int)fdk_cosine_weight);
+    };
+    if (kernel == AP_RAMP_RAMLAK) launch(k_ramp_rows<true>);
+    else launch(k_ramp_rows<false>);
     return ap_check_launch("ramp_filter_rows");
 }

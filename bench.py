@@ -19,7 +19,13 @@ the reference semantics timed on a bounded sample of the same workload.
 roofline.traffic (HBM bytes per launch of the dominant kernel) is read from
 profiles/hbm_traffic_siddon_512cube_720views.txt, the summary of two
 rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters
-read for --calibrate-copy; it is null when that file is absent.
+read for --calibrate-copy; it is null when that file is absent.  With it,
+roofline.measured_gbs = traffic / kernel time.  roofline.valu_issue_frac is
+SQ_INSTS_VALU per launch (profiles/sq_instruction_mix_siddon_512cube_720views.md)
+over the VALU issue capacity in the kernel's time (one wave64 VALU
+instruction per CU per cycle, 256 CUs at 2.4 GHz), and roofline.bound names
+the larger of the two fractions ("valu" or "hbm"); with no counter file the
+bound stays the per-view model's "hbm".
 """
 
 from __future__ import annotations
@@ -36,6 +42,11 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GUPS fwd+adjoint proj at 512\u00b3\u00d7720 views"
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "hbm_traffic_siddon_512cube_720views.txt")
+SQ_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                       "sq_instruction_mix_siddon_512cube_720views.md")
+# VALU issue capacity: each CU issues one wave64 VALU instruction per cycle
+# (4 SIMDs, 4 cycles each); MI355X: 256 CUs, 2.4 GHz peak engine clock
+VALU_ISSUE_PER_S = 256 * 2.4e9
 
 
 def log(msg: str) -> None:
@@ -148,6 +159,16 @@ def fdk_leg(reps: int = 3) -> dict:
                                  approx_mode_bilinear_interpolate=True)
     vol = Volume.zeros(truth.shape, voxwid=w)
     stream = torch.cuda.current_stream()
+    # the backprojector's launch shape for this problem, timed on a scratch volume
+    from aprict_amd import _abi
+    from aprict_amd.autotune import Autotuner
+
+    tuner = Autotuner(repeats=2)
+    ps_b = cbfp.ProjectionSet(pg, data=b)
+    scratch = Volume.zeros(truth.shape, voxwid=w)
+    key = ("fdk", n, V, c["nu"], c["nv"])
+    tuner.tune(_abi.TUNE_BACKPROJECT, key, lambda: cbfp.backproject(cfg, scratch, ps_b))
+    del scratch, ps_b
     tf, tb = [], []
     for _ in range(reps + 1):
         vol.data.zero_()
@@ -165,7 +186,9 @@ def fdk_leg(reps: int = 3) -> dict:
     return {"workload": f"{n}^3 ({w} mm), {V} views, {c['nu']}x{c['nv']}, cosine weight + Ram-Lak + direct "
                         "ray-casting backprojection (Z/R)^-2", "ms_filter": t_f * 1e3, "ms_backproject": t_b * 1e3,
             "ms_total": (t_f + t_b) * 1e3, "gups_backproject": float(n) ** 3 * V / t_b / 1e9,
-            "rel_l2_central_slice_vs_phantom": float((got - want).norm() / want.norm())}
+            "rel_l2_central_slice_vs_phantom": float((got - want).norm() / want.norm()),
+            "tuned": {"backproject": {"variant": tuner.best(_abi.TUNE_BACKPROJECT, key),
+                                      "times_ms": tuner.times_ms(_abi.TUNE_BACKPROJECT, key)}}}
 
 
 def sirt_seconds_per_iter(iters: int = 20) -> dict:
@@ -402,18 +425,32 @@ def tv_gd_leg(P, truth, iters: int = 5, power_iterations: int = 8) -> dict:
     return out
 
 
-def read_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (a line 'traffic_bytes_per_launch <kernel> <fetch> <write>')."""
+def _summary_line(path: str, tag: str, kernel: str, nvals: int):
+    """The numbers of the line '<tag> <kernel> v1 .. vn' of a committed
+    rocprofv3 summary, or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                    return float(parts[2]) + float(parts[3])
+                if len(parts) == 2 + nvals and parts[0] == tag and parts[1] == kernel:
+                    return [float(v) for v in parts[2:]]
     except OSError:
         return None
     return None
+
+
+def read_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` (fetch + write) from the committed
+    rocprofv3 summary (a line 'traffic_bytes_per_launch <kernel> <fetch> <write>')."""
+    v = _summary_line(TRAFFIC_FILE, "traffic_bytes_per_launch", kernel, 2)
+    return None if v is None else v[0] + v[1]
+
+
+def read_valu(kernel: str):
+    """SQ_INSTS_VALU per launch of `kernel` from the committed rocprofv3
+    summary (a line 'sq_insts_valu_per_launch <kernel> <count>')."""
+    v = _summary_line(SQ_FILE, "sq_insts_valu_per_launch", kernel, 1)
+    return None if v is None else v[0]
 
 
 def calibrate_copy() -> int:
@@ -518,10 +555,16 @@ def main() -> int:
     s_v = 2 if args.volume_dtype == "bf16" else 4
     b_op = V * (n**3 * s_v + nu * nv * 4)
     dom = "adjoint" if t_at >= t_a else "forward"
-    dom_kernel = "k_backproject_columns" if dom == "adjoint" else "k_project_columns"
+    dom_kernel = "k_gather_adjoint" if dom == "adjoint" else "k_project_columns"
     t_dom = max(t_a, t_at)
     achieved = b_op / t_dom / 1e9
     traffic = read_traffic(dom_kernel) if args.volume_dtype == "f32" else None
+    valu = read_valu(dom_kernel) if args.volume_dtype == "f32" else None
+    measured_gbs = None if traffic is None else traffic / t_dom / 1e9
+    valu_frac = None if valu is None else valu / (VALU_ISSUE_PER_S * t_dom)
+    bound = "hbm"
+    if valu_frac is not None and measured_gbs is not None and valu_frac > measured_gbs / HBM_PEAK_GBS:
+        bound = "valu"
     out = {
         "metric": METRIC,
         "value": gups,
@@ -544,10 +587,16 @@ def main() -> int:
         "gups_adjoint": voxel_views / t_at / 1e9,
         "dot_test_rel": dot_rel,
         "tuned": tuned,
-        "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": bound, "kernel": dom_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "model": "B_op = V (N^3 s_v + nu nv s_p) bytes per operator application; time = the "
-                              "operator call on its stream (HIP events)"},
+                     "measured_gbs": measured_gbs,
+                     "measured_frac": None if measured_gbs is None else measured_gbs / HBM_PEAK_GBS,
+                     "valu_issue_frac": valu_frac,
+                     "model": "achieved = B_op / t with B_op = V (N^3 s_v + nu nv s_p) bytes per operator "
+                              "application and t = the operator call on its stream (HIP events); "
+                              "measured_gbs = rocprofv3 FETCH+WRITE bytes per launch / t; valu_issue_frac = "
+                              "SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x t); bound = the larger of "
+                              "measured_frac and valu_issue_frac"},
     }
     if args.volume_dtype == "f32" and not args.no_storage:
         log("16-bit storage leg")

@@ -414,12 +414,12 @@ int32_t ap_sirt_iterate(const ap_config_cb_backproject* bp_cfg,
 /* ---- launch-shape tuning -------------------------------------------------
  * No counterpart in the reference (its launch shapes are fixed).  For the
  * kernels whose best launch shape depends on the problem size, the library
- * holds one process-wide variant index per operation; every variant computes
- * bit-identical results, only the work decomposition differs.  The
- * project-written autotuner (aprict_amd/autotune.py) times the variants per
- * problem class and selects one before launching (also while a HIP graph is
- * being captured: the variant is read when the launch is recorded).  Not
- * synchronised across host threads: set it from the thread that launches.
+ * holds one variant index per operation and per host thread (a call reads
+ * the calling thread's selection); every variant computes bit-identical
+ * results, only the work decomposition differs.  The project-written
+ * autotuner (aprict_amd/autotune.py) times the variants per problem class
+ * and selects one before launching (also while a HIP graph is being
+ * captured: the variant is read when the launch is recorded).
  *   AP_TUNE_PROJECT_RAY     per-ray forward (tilted detectors): pixels per
  *                           workgroup 64x4, 64x2, 64x1 (gathers per batch
  *                           4, 4, 8)

@@ -13,13 +13,16 @@ Arrays are dense C-order numpy arrays: volumes [Nx][Ny][Nz], sinograms
 from __future__ import annotations
 
 import ctypes
+import os
 import pathlib
 import subprocess
 
 import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
-BUILD = HERE / "_build"
+# ORACLE_BUILD selects another build of the same sources (e.g. _build_san,
+# the AddressSanitizer/UBSan build of `make sanitize`)
+BUILD = HERE / os.environ.get("ORACLE_BUILD", "_build")
 
 _P = ctypes.c_void_p
 _LIBS: dict[bool, ctypes.CDLL] = {}

@@ -140,3 +140,23 @@ def test_python_layer_refuses_cpu_tensors():
     ps = cbfp.ProjectionSet(pg, data=torch.zeros(2, 3, 3), device="cpu")
     with pytest.raises(ValueError, match="GPU"):
         cbfp.project(cbfp.ConfigProject(), v, ps)
+
+
+def test_launch_shape_variant_is_per_host_thread():
+    import threading
+
+    lib = aprict_amd.lib()
+    assert lib.ap_tune_set(_abi.TUNE_PROJECT_COLUMN, 2) == 0
+    seen = {}
+
+    def other():
+        seen["before"] = lib.ap_tune_get(_abi.TUNE_PROJECT_COLUMN)
+        lib.ap_tune_set(_abi.TUNE_PROJECT_COLUMN, 1)
+        seen["after"] = lib.ap_tune_get(_abi.TUNE_PROJECT_COLUMN)
+
+    t = threading.Thread(target=other)
+    t.start()
+    t.join()
+    assert seen == {"before": 0, "after": 1}
+    assert lib.ap_tune_get(_abi.TUNE_PROJECT_COLUMN) == 2
+    assert lib.ap_tune_set(_abi.TUNE_PROJECT_COLUMN, 0) == 0

@@ -54,6 +54,21 @@ def test_ramp_filter_matches_oracle(device, fdk):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3], ids=["ramlak", "sinc", "hilbert", "hilbert_half"])
+def test_ramp_filter_wide_rows_and_every_kernel(device, kernel):
+    # rows longer than any LDS tap table (the taps are staged per input chunk),
+    # odd lengths and a last block of partial outputs; each kernel of
+    # ramp_kernels.py against the oracle's direct convolution
+    rng = np.random.default_rng(30 + kernel)
+    nu = 5003 if kernel == 0 else 1037
+    x = rng.standard_normal((2, nu, 40)).astype(np.float32)
+    got = rf.ramp_filter_rows(to_gpu(x), 0.3, kernel=kernel).cpu().numpy()
+    want = orc.ramp_filter(x, 0.3, kernel=kernel)
+    err = rel_l2(got, want)
+    print(f"ramp kernel {kernel}, nu {nu}: rel L2 {err:.3e}")
+    assert err <= 1e-5
+
+
 def test_ramp_filterer_api(device):
     rng = np.random.default_rng(4)
     x = rng.standard_normal((7, 40, 12)).astype(np.float32)
