@@ -15,7 +15,8 @@ struct GView {
     float ax, ay;        // detector point of sub-column 0 minus the source (x, y)
     float bx, by;        // step of the detector point per sub-column (x, y)
     float orient;        // sign of cross(b, a): cross(b, X - S) * orient > 0 in front
-    float pad[4];
+    int32_t t0;          // rows (z order) whose ray does not rise (dz <= 0); set by the gather pre-pass
+    float pad[3];
 };
 static_assert(sizeof(GView) == 64, "GView layout");
 
@@ -65,6 +66,7 @@ __device__ inline bool gather_view(const RayCfg& rc, const ap_proj_spec* __restr
     gv.bx = (float)bx;
     gv.by = (float)by;
     gv.orient = (float)orient;
+    gv.t0 = 0;
     return true;
 }
 

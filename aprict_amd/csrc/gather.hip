@@ -29,15 +29,20 @@
 // prefix sums P of the sec-weighted sinogram column (in z order), the
 // column's x-y crossing constants (the forward's col_setup, so the cell
 // boundaries along h are the forward's own FP32 values) and, per view, the
-// reciprocal heights 1/dz_t.  The main kernel (k_gather_adjoint) gives each
-// workgroup a tile of x-y cells and a z range of 64*kJ boundaries; each wave
-// owns kC cells, each lane one boundary per 64 in z.  Per view the workgroup
-// stages the prefix window of the columns crossing the tile into LDS, each
-// wave derives its (cell, column) pairs one per lane, then for every pair
-// every lane evaluates G at its boundaries (about 20 VALU operations, two
-// LDS reads) and adds it to its FP64 accumulators.  The voxel values are
-// differences of neighbouring boundaries, scaled and added to the volume
-// once at the end.
+// reciprocal heights 1/dz_t and the number t0 of rows whose ray does not
+// rise.  The main kernel (k_gather_adjoint) gives each workgroup (16 waves)
+// a tile of 8x8 x-y cells and a z range of 64*kJ boundaries; each wave owns
+// kC cells, each lane one boundary per 64 in z.  The work comes in items, a
+// view and a group of the columns crossing the tile: while the workgroup
+// works on one item, the next one's prefix window is loaded straight into
+// the other of two LDS buffers (16-byte loads).  Per item each wave derives
+// its (cell, column) pairs one per lane, then for every pair every lane
+// evaluates G at its boundaries (14 VALU operations and two LDS reads on
+// tiles that lie on one side of the source plane, 18 otherwise) and adds it
+// to its FP64 accumulators.  The voxel values are differences of
+// neighbouring boundaries, scaled and added to the volume once at the end.
+// Tiles run in blocks of 4x8, so that the workgroups on one XCD at one time
+// share their views' prefix windows through its L2.
 //
 // Precision: the sums P and the accumulators are FP64, so a voxel's value is
 // exact to the FP32 rounding of its own terms whatever the dynamic range of
@@ -54,9 +59,9 @@
 namespace {
 
 constexpr int kXcdG = 8;
-constexpr int kGCap = 4608;        // FP64 entries of the LDS prefix window (36 KB)
+constexpr int kGWin = 57344;       // bytes of an LDS window buffer (56 KB; two per workgroup)
 constexpr int kGMaxCols = 16;      // columns staged at once
-constexpr int kGRowMax = 2048;     // rows of the window (1/dz table)
+constexpr int kGRowMax = 2048;     // rows of a window (at most)
 
 // per-(view, sub-column) record
 struct GCol {
@@ -183,18 +188,25 @@ __global__ __launch_bounds__(256) void k_gather_prep(RayCfg rc, GCfg gc, const a
         g.pad = 0.f;
         gcols[w] = g;
     }
-    // 1/dz per row (0 past the last ray), once per view
+    // 1/dz per row (0 past the last ray), once per view; and the number of
+    // rows whose ray does not rise (dz <= 0: the rows before the first rising
+    // one, since dz grows along the z order)
     if (q == 0) {
+        uint32_t nfall = 0;
         for (uint32_t t = (uint32_t)lane; t < gc.cs; t += 64) {
             double v = 0.0;
+            bool fall = false;
             if (t < NT) {
                 const uint32_t sr = gv.flip ? NT - 1 - t : t;
                 const float kv = (float)(sr / gc.ssv) + sub_offset_g(sr % gc.ssv, gc.ssv);
                 const double dz = vr.Q00[2] + (double)kv * vr.V[2] - vr.S[2];
                 v = dz != 0.0 ? 1.0 / dz : 0.0;
+                fall = !(dz > 0.0);
             }
             izt[(size_t)p * gc.cs + t] = v;
+            nfall += (uint32_t)__popcll(__ballot(fall));
         }
+        if (lane == 0) gviews[p].t0 = (int32_t)nfall;
     }
 }
 
@@ -213,8 +225,15 @@ __global__ __launch_bounds__(256) void k_gather_count(const GView* __restrict__ 
 // ---------------------------------------------------------------------------
 // main kernel
 // ---------------------------------------------------------------------------
+// Tiles are taken in blocks of kSTX x kSTY (x, y) tiles of one z layer: the
+// workgroups that run at the same time on one XCD cover a compact block of
+// cells, whose views' prefix windows they share through that XCD's L2.
+constexpr uint32_t kSTX = 4, kSTY = 8, kST = kSTX * kSTY;
+
 struct GTile {
-    uint32_t ntx, nty, ntz, total, per_xcd;
+    uint32_t ntx, nty, ntz;   // tiles along x, y, z
+    uint32_t nsx, nsy;        // tile blocks along x, y
+    uint32_t total, per_xcd;  // block slots (ntz nsx nsy kST), slots per XCD
 };
 
 // integer part of x, one instruction (x within int range)
@@ -235,27 +254,43 @@ struct GWin {
 };
 
 template <typename Tv, typename Tm, int kW, int kC, int kJ, int kTX>
-__global__ __launch_bounds__(kW * 64, 4) void k_gather_adjoint(
+__global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
     RayCfg rc, GCfg gc, GTile tl, const GView* __restrict__ gviews, const GCol* __restrict__ gcols,
     const double* __restrict__ pref, const double* __restrict__ izt, double scale, Tv* __restrict__ vol,
     int64_t vsx, int64_t vsy, int64_t vsz, const Tm* __restrict__ mult, int64_t msx, int64_t msy, int64_t msz) {
     constexpr int kCells = kW * kC;
     constexpr int kTY = kCells / kTX;
-    constexpr int kT = kW * 64;
+    constexpr int kNQc = (kTX + 1) * (kTY + 1);
     static_assert(kTX * kTY == kCells, "tile shape");
     static_assert(64 / kC >= kGMaxCols, "one setup pass per group");
-    __shared__ double Pw[kGCap];
-    __shared__ double Iw[kGRowMax];
-    __shared__ __align__(16) GCol Cw[kGMaxCols];   // 16 records = 5 whole chunks
-    __shared__ float Qc[(kTX + 1) * (kTY + 1)];
+    // Two staging buffers: while the workgroup works on one (view, column group)
+    // item from one buffer, the next item's window is loaded into the other.
+    // A buffer holds the item's window -- rows tlo .. tlo + Wp - 1 of 1/dz, then
+    // the same rows of P for each column of the group --, the group's column
+    // records and the projections of the tile's cell corners.
+    struct GBuf {
+        double win[kGWin / 8];
+        GCol cols[kGMaxCols];
+        float qc[(kNQc + 3) & ~3];
+    };
+    __shared__ __align__(16) GBuf Buf0;
+    __shared__ __align__(16) GBuf Buf1;
     __shared__ GWin Win[64];
     // per-wave (cell, column) pair records of the setup pass, one per lane:
-    // h_a, h_b - h_a, alpha_a, alpha_b, l, window offset | flags, hc / l (FP64)
-    __shared__ __align__(16) float Prec[kW][64][8];
+    // h_a, h_b - h_a, alpha_a, alpha_b, l, window offset | flags, hc / l (FP64).
+    // The records start 8 bytes past a 16-byte boundary: the compiler then reads
+    // them as 8-byte pairs, which keep its knowledge that they do not touch the
+    // window loads in flight (its 16-byte reads lose it and wait for them)
+    __shared__ __align__(16) float PrecA[kW * 64 * 8 + 2];
+    auto prec = [&](int w, int l) -> float* { return PrecA + 2 + (w * 64 + l) * 8; };
     const uint32_t L = blockIdx.x;
     const uint32_t logical = (L % kXcdG) * tl.per_xcd + L / kXcdG;
     if (logical >= tl.total) return;
-    const uint32_t tx = logical % tl.ntx, ty = (logical / tl.ntx) % tl.nty, tz = logical / (tl.ntx * tl.nty);
+    const uint32_t per_layer = tl.nsx * tl.nsy * kST;
+    const uint32_t tz = logical / per_layer, r = logical % per_layer;
+    const uint32_t st = r / kST, w = r % kST;
+    const uint32_t tx = (st % tl.nsx) * kSTX + w % kSTX, ty = (st / tl.nsx) * kSTY + w / kSTX;
+    if (tx >= tl.ntx || ty >= tl.nty) return;   // a slot of a block past the edge
     const int X0 = rc.lo[0] + (int)tx * kTX, Y0 = rc.lo[1] + (int)ty * kTY;
     const int X1 = min(X0 + kTX, rc.hi[0]), Y1 = min(Y0 + kTY, rc.hi[1]);
     const int z0 = rc.lo[2] + (int)tz * 64 * kJ;
@@ -286,6 +321,281 @@ __global__ __launch_bounds__(kW * 64, 4) void k_gather_adjoint(
         for (int j = 0; j < kJ; j++) acc[i][j] = 0.0;
     double topacc = 0.0;   // G at z1 of this lane's setup cell (cell si)
     const float zf1 = (float)z1;
+
+    // load item (view p, columns g0 .. g0 + ng - 1) into buffer B: 1/dz and P
+    // rows in 1 KB pieces (16 bytes per lane, straight into LDS), the last piece
+    // of each row set with only the lanes it needs
+    auto stage = [&](uint32_t p, const GWin& wn, int g0, int ng, GBuf& B) {
+        // the cell corners' projections (column coordinate), written before the
+        // loads into LDS are issued (no wait on them)
+        if (tid < kNQc) {
+            const GView g = gviews[p];
+            const int cx = tid % (kTX + 1), cy = tid / (kTX + 1);
+            const float dx = (float)min(X0 + cx, rc.hi[0]) - g.sx;
+            const float dy = (float)min(Y0 + cy, rc.hi[1]) - g.sy;
+            B.qc[tid] = -(g.ax * dy - g.ay * dx) * frcp(g.bx * dy - g.by * dx);
+        }
+        const int Wp = (wn.W + 31) & ~31;
+        const int bytes = Wp * 8;                  // per row set (a multiple of 256)
+        const int npc = (bytes + 1023) / 1024;     // pieces per row set
+        const int lastl = (bytes - (npc - 1) * 1024) / 16;   // lanes of the last piece
+        // the lane's byte offsets, opaque to the compiler so that the 64-bit
+        // staging addresses are formed here, not held in registers across items
+        uint32_t l16 = (uint32_t)lane * 16, l4 = (uint32_t)lane * 4;
+        asm volatile("" : "+v"(l16), "+v"(l4));
+        char* dst0 = reinterpret_cast<char*>(B.win);
+        const char* izp = reinterpret_cast<const char*>(izt + (uint64_t)p * gc.cs + wn.tlo) + l16;
+        const char* colp = reinterpret_cast<const char*>(pref + ((uint64_t)p * gc.NQ + g0) * gc.cs + wn.tlo) + l16;
+        // row set 0: 1/dz; row set c + 1: P of column g0 + c
+        for (int e = wave; e < (ng + 1) * npc; e += kW) {
+            const int c = e / npc, i = e - c * npc;
+            const char* src = c == 0 ? izp + i * 1024 : colp + (uint64_t)(c - 1) * gc.cs * 8 + i * 1024;
+            if (i + 1 < npc || lane < lastl)
+                __builtin_amdgcn_global_load_lds(src, dst0 + c * bytes + i * 1024, 16, 0, 0);
+        }
+        // the group's column records (80 B each)
+        for (int ch = wave; ch * 64 < ng * 20; ch += kW) {
+            const char* src = reinterpret_cast<const char*>(gcols + (uint64_t)p * gc.NQ + g0) + ch * 256 + l4;
+            __builtin_amdgcn_global_load_lds(src, reinterpret_cast<char*>(B.cols) + ch * 256, 4, 0, 0);
+        }
+    };
+
+    // the setup and main passes of item (p, g0 .. g1) from buffer B
+    auto work = [&](uint32_t p, const GWin& wn, int g0, int g1, const GBuf& B) {
+        const GView g = gviews[p];
+        const int W = wn.W, tlo = wn.tlo;
+        const int Wp = (W + 31) & ~31;
+        // k - sz per slot, exact in FP64 (k an integer below 2^24, sz a float)
+        double kzd[kJ];
+#pragma unroll
+        for (int jj = 0; jj < kJ; jj++) kzd[jj] = (double)(z0 + lane + 64 * jj) - (double)g.sz;
+        const double kz1d = (double)z1 - (double)g.sz;
+        const float kzspan = fmaxf(fabsf((float)z0 - g.sz), fabsf(zf1 - g.sz));
+        const float Wm2 = (float)(W - 2);
+        const float msz = -g.sz;
+        // When every boundary of the tile lies at or above the source (z0 >= sz), a
+        // ray that does not rise is entirely below each plane k, and the ray tf that
+        // reaches plane k inside a cell rises: tf >= t0 (the first rising row), F is
+        // the part before the crossing, and kappa_a - kappa_b = (k - sz)(alpha_a -
+        // alpha_b) has one sign.  At or below the source (z1 - 1 <= sz) the same holds
+        // for falling rays: tf < t0 and F is the part after the crossing.  Either
+        // way one FMA gives kappa and the clamp of tf carries the side.  The window
+        // must hold a row of that side.
+        const int t0w = g.t0 - tlo;
+        const bool upv = (float)z0 - g.sz >= 0.f && t0w <= W - 2;
+        const bool dnv = !upv && (float)(z1 - 1) - g.sz <= 0.f && t0w >= 1;
+        const float tlo_up = (float)max(0, t0w), thi_dn = (float)min(W - 2, t0w - 1);
+        const char* Pb = reinterpret_cast<const char*>(B.win);   // 1/dz rows at 0, column c at (c + 1) Wp 8
+        const double* Iw = B.win;
+        // ---- setup pass: lane (si, ss) takes column qa(cell si) + ss ----
+        float ha = 0.f, dh = 0.f, aa = 0.f, ba = 0.f, ab = 0.f, bbt = 0.f, lq = 0.f, yhd = 0.f, ifh = 0.f;
+        double gm = 0.0;
+        int pb_ = 0, mp = 0, pcol = 0;
+        bool valid = false;
+        {
+            const float c00 = B.qc[s_ly * (kTX + 1) + s_lx], c10 = B.qc[s_ly * (kTX + 1) + s_lx + 1];
+            const float c01 = B.qc[(s_ly + 1) * (kTX + 1) + s_lx], c11 = B.qc[(s_ly + 1) * (kTX + 1) + s_lx + 1];
+            const float cmin = fminf(fminf(c00, c10), fminf(c01, c11));
+            const float cmax = fmaxf(fmaxf(c00, c10), fmaxf(c01, c11));
+            const int qa = max(g0, (int)floorf(cmin - 1e-3f));
+            const int qb = min(g1, (int)ceilf(cmax + 1e-3f));
+            const int q = qa + ss;
+            if (s_cv && q <= qb) {
+                const GCol& G = B.cols[q - g0];
+                float h0 = G.hs, h1 = G.hd;
+                const int blo[2] = {s_ci, s_cj}, bhi[2] = {s_ci + 1, s_cj + 1};
+#pragma unroll
+                for (int a = 0; a < 2; a++) {
+                    if (G.sg[a] == 0) {
+                        if (G.ifix[a] != blo[a]) h1 = -AP_BIG;
+                        continue;
+                    }
+                    const float ein = (float)(G.sg[a] > 0 ? blo[a] : bhi[a]) - G.P0[a];
+                    const float eout = (float)(G.sg[a] > 0 ? bhi[a] : blo[a]) - G.P0[a];
+                    h0 = fmaxf(h0, tcross(ein, G.r[a], G.c[a]));
+                    h1 = fminf(h1, tcross(eout, G.r[a], G.c[a]));
+                }
+                if (h0 < h1) {
+                    valid = true;
+                    ha = h0;
+                    dh = h1 - h0;
+                    aa = G.lods * frcp(h0 + G.hc);
+                    ab = G.lods * frcp(h1 + G.hc);
+                    ba = __builtin_fmaf(-g.sz, aa, wn.kc);
+                    bbt = __builtin_fmaf(-g.sz, ab, wn.kc);
+                    lq = G.l;
+                    gm = G.gam;
+                    pb_ = (q - g0 + 1) * Wp * 8;
+                    pcol = q - g0;
+                    yhd = G.yh * dh;
+                    ifh = (float)G.ifixh;
+                    // rays straddling one plane inside the cell lie in
+                    // (min kappa, max kappa], |kappa_a - kappa_b| = |k - sz| |alpha_a - alpha_b|:
+                    // more than one of them needs the general evaluation
+                    mp = kzspan * fabsf(aa - ab) >= 0.99f ? 1 : 0;
+                }
+            }
+        }
+        // G(k) of one (cell, column) pair at boundary kf (kz = kf - sz in FP64), added to a
+        auto gacc = [&](double& a, float kf, double kz, float ha_, float dh_, float aa_, float ba_, float ab_,
+                        float bb_, float l_, double gm_, int pbb) {
+            const float ka = __builtin_fmaf(kf, aa_, ba_), kb = __builtin_fmaf(kf, ab_, bb_);
+            const int tf = cvt_flr(__builtin_amdgcn_fmed3f(fminf(ka, kb), 0.f, Wm2));
+            const double P0 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8);
+            const double P1 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8 + 8);
+            const double iz = Iw[tf];
+            // crossing of plane k by ray tf, along h - hc, relative to the cell entry
+            const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
+            const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
+            const float F = iz > 0.0 ? Fp : dh_ - Fp;
+            a = fma((double)dh_, P0, a);
+            a = fma((double)F, P1 - P0, a);
+        };
+        // the same with every ray that may straddle plane k
+        auto gacc_multi = [&](double& a, float kf, double kz, float ha_, float dh_, float aa_, float ba_,
+                              float ab_, float bb_, float l_, double gm_, int pbb) {
+            const float ka = __builtin_fmaf(kf, aa_, ba_), kb = __builtin_fmaf(kf, ab_, bb_);
+            const int tf = cvt_flr(__builtin_amdgcn_fmed3f(fminf(ka, kb), 0.f, Wm2));
+            const int te = cvt_flr(__builtin_amdgcn_fmed3f(fmaxf(ka, kb), (float)tf, Wm2));
+            a = fma((double)dh_, *reinterpret_cast<const double*>(Pb + pbb + tf * 8), a);
+            for (int t = tf; t <= te; t++) {
+                const double iz = Iw[t];
+                const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
+                const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
+                const float F = iz > 0.0 ? Fp : dh_ - Fp;
+                const double P0 = *reinterpret_cast<const double*>(Pb + pbb + t * 8);
+                const double P1 = *reinterpret_cast<const double*>(Pb + pbb + t * 8 + 8);
+                a = fma((double)F, P1 - P0, a);
+            }
+        };
+        // G(k) on one side of the source (kap: kappa at the end that is the smaller
+        // at every boundary of the tile); rising rays: dh P[tf] + F (P[tf+1] - P[tf]),
+        // falling rays: dh P[tf+1] - Fp (P[tf+1] - P[tf]) with F = dh - Fp
+        auto gup = [&](double& a, float kap, double kz, float ha_, float dh_, float l_, double gm_, int pbb) {
+            const int tf = cvt_flr(__builtin_amdgcn_fmed3f(kap, tlo_up, Wm2));
+            const double P0 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8);
+            const double P1 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8 + 8);
+            const double iz = Iw[tf];
+            const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
+            const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
+            a = fma((double)dh_, P0, a);
+            a = fma((double)Fp, P1 - P0, a);
+        };
+        auto gdn = [&](double& a, float kap, double kz, float ha_, float dh_, float l_, double gm_, int pbb) {
+            const int tf = cvt_flr(__builtin_amdgcn_fmed3f(kap, 0.f, thi_dn));
+            const double P0 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8);
+            const double P1 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8 + 8);
+            const double iz = Iw[tf];
+            const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
+            const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
+            a = fma((double)dh_, P1, a);
+            a = fma(-(double)Fp, P1 - P0, a);
+        };
+        {
+            // flags: bit 24 several rays may straddle a plane, bit 25 the
+            // column has a ray lying in a z plane, bits 26-29 its column in
+            // the group; bits 0-23 the byte offset of its rows in the window
+            float* R = prec(wave, lane);
+            const uint64_t gbits = (uint64_t)__double_as_longlong(gm);
+            const int flags = pb_ | (mp << 24) | ((yhd != 0.f ? 1 : 0) << 25) | (pcol << 26);
+            reinterpret_cast<float2*>(R)[0] = make_float2(ha, dh);
+            reinterpret_cast<float2*>(R)[1] = make_float2(aa, ab);
+            reinterpret_cast<float2*>(R)[2] = make_float2(lq, __int_as_float(flags));
+            reinterpret_cast<float2*>(R)[3] =
+                make_float2(__uint_as_float((uint32_t)gbits), __uint_as_float((uint32_t)(gbits >> 32)));
+        }
+        if (valid) {
+            double gt = 0.0;
+            if (mp)
+                gacc_multi(gt, zf1, kz1d, ha, dh, aa, ba, ab, bbt, lq, gm, pb_);
+            else
+                gacc(gt, zf1, kz1d, ha, dh, aa, ba, ab, bbt, lq, gm, pb_);
+            if (yhd != 0.f && zf1 > ifh) gt += (double)yhd;
+            topacc += gt;
+        }
+        // ---- main pass: every lane, every pair of cell i ----
+#pragma unroll
+        for (int i = 0; i < kC; i++) {
+            uint64_t m = __ballot(valid && si == i);
+            while (m) {
+                const int src = __builtin_ctzll(m);
+                m &= m - 1;
+                // the pair's record, read by every lane at one address (a broadcast).
+                // The read is written out, with its wait in the same block, so that
+                // the compiler does not also wait for the next item's window loads
+                // (its alias information does not reach every LDS read)
+                const uint32_t ra = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)prec(wave, src);
+                float4 q0, q1;
+                asm volatile(
+                    "ds_read2_b64 %0, %2 offset1:1\n\t"
+                    "ds_read2_b64 %1, %2 offset0:2 offset1:3\n\t"
+                    "s_waitcnt lgkmcnt(0)"
+                    : "=&v"(q0), "=&v"(q1)   // early clobber: no output shares the address register
+                    : "v"(ra)
+                    : "memory");
+                const float rha = q0.x, rdh = q0.y, raa = q0.z, rab = q0.w;
+                const float rl = q1.x;
+                const int rfl = __float_as_int(q1.y);
+                const int rpb = rfl & 0xFFFFFF;
+                const double rgm = __longlong_as_double(
+                    (long long)(((uint64_t)__float_as_uint(q1.w) << 32) | __float_as_uint(q1.z)));
+                if (rfl & (1 << 24)) {
+                    const float rba = __builtin_fmaf(msz, raa, wn.kc), rbb = __builtin_fmaf(msz, rab, wn.kc);
+                    for (int jj = 0; jj < kJ; jj++) {
+                        const float kf = (float)(z0 + lane + 64 * jj);
+                        gacc_multi(acc[i][jj], kf, kzd[jj], rha, rdh, raa, rba, rab, rbb, rl, rgm, rpb);
+                    }
+                } else if (upv) {
+                    // kappa_a - kappa_b has the sign of alpha_a - alpha_b
+                    const float ram = raa <= rab ? raa : rab;
+                    const float rbm = __builtin_fmaf(msz, ram, wn.kc);
+#pragma unroll
+                    for (int jj = 0; jj < kJ; jj++)
+                        gup(acc[i][jj], __builtin_fmaf((float)(z0 + lane + 64 * jj), ram, rbm), kzd[jj], rha, rdh, rl,
+                            rgm, rpb);
+                } else if (dnv) {
+                    // kappa_a - kappa_b has the sign of alpha_b - alpha_a
+                    const float ram = raa >= rab ? raa : rab;
+                    const float rbm = __builtin_fmaf(msz, ram, wn.kc);
+#pragma unroll
+                    for (int jj = 0; jj < kJ; jj++)
+                        gdn(acc[i][jj], __builtin_fmaf((float)(z0 + lane + 64 * jj), ram, rbm), kzd[jj], rha, rdh, rl,
+                            rgm, rpb);
+                } else {
+                    const float rba = __builtin_fmaf(msz, raa, wn.kc), rbb = __builtin_fmaf(msz, rab, wn.kc);
+#pragma unroll
+                    for (int jj = 0; jj < kJ; jj++)
+                        gacc(acc[i][jj], (float)(z0 + lane + 64 * jj), kzd[jj], rha, rdh, raa, rba, rab, rbb, rl, rgm,
+                             rpb);
+                }
+                if (rfl & (1 << 25)) {
+                    // rays lying in a z plane: their voxel row is below the plane
+                    const GCol& G = B.cols[(rfl >> 26) & 15];
+                    const double ryh = (double)(G.yh * rdh);
+                    const float rif = (float)G.ifixh;
+#pragma unroll
+                    for (int jj = 0; jj < kJ; jj++)
+                        if ((float)(z0 + lane + 64 * jj) > rif) acc[i][jj] += ryh;
+                }
+            }
+        }
+    };
+
+    // a window record read as wave-uniform values (scalar registers), so that the
+    // view records read with them are scalar loads: a vector load in the work
+    // phase would wait for the in-flight staging loads (one in-order counter)
+    auto win_at = [&](int idx) -> GWin {
+        const GWin w = Win[idx];
+        GWin u;
+        u.qlo = __builtin_amdgcn_readfirstlane(w.qlo);
+        u.qhi = __builtin_amdgcn_readfirstlane(w.qhi);
+        u.tlo = __builtin_amdgcn_readfirstlane(w.tlo);
+        u.W = __builtin_amdgcn_readfirstlane(w.W);
+        u.kc = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(w.kc)));
+        u.ncg = __builtin_amdgcn_readfirstlane(w.ncg);
+        u.pad[0] = u.pad[1] = 0;
+        return u;
+    };
 
     for (uint32_t pb = 0; pb < gc.V; pb += 64) {
         // ---- windows of views pb .. pb + 63, one per lane of wave 0 ----
@@ -336,207 +646,58 @@ __global__ __launch_bounds__(kW * 64, 4) void k_gather_adjoint(
                     wn.tlo = tlo;
                     wn.W = thi - tlo + 2;   // P[tlo .. thi + 1]
                     wn.kc = 1.f - (float)tlo - g.dz0 / g.dzs;
-                    wn.ncg = min(kGMaxCols, kGCap / ((wn.W + 31) & ~31));
+                    // columns per group: the window holds Wp rows of 1/dz and of P per column
+                    const int Wp = (wn.W + 31) & ~31;
+                    wn.ncg = min(kGMaxCols, kGWin / (Wp * 8) - 1);
                 }
             }
             Win[lane] = wn;
         }
         __syncthreads();
         const uint32_t pe = min(gc.V, pb + 64);
-        for (uint32_t p = pb; p < pe; p++) {
-            const GWin wn = Win[p - pb];
-            if (wn.qlo > wn.qhi) continue;   // uniform: the view misses the tile (or is not handled here)
-            const GView g = gviews[p];
-            const int W = wn.W, tlo = wn.tlo;
-            const int Wp = (W + 31) & ~31;   // column stride of the window, whole 256-byte chunks
-            const uint64_t colbase = (uint64_t)p * gc.NQ;
-            // k - sz per slot, exact in FP64 (k an integer below 2^24, sz a float)
-            double kzd[kJ];
-#pragma unroll
-            for (int jj = 0; jj < kJ; jj++) kzd[jj] = (double)(z0 + lane + 64 * jj) - (double)g.sz;
-            const double kz1d = (double)z1 - (double)g.sz;
-            const float kzspan = fmaxf(fabsf((float)z0 - g.sz), fabsf(zf1 - g.sz));
-            const float Wm2 = (float)(W - 2);
-            const float msz = -g.sz;
-            for (int g0 = wn.qlo; g0 <= wn.qhi; g0 += wn.ncg) {
-                const int g1 = min(wn.qhi, g0 + wn.ncg - 1);
-                const int ng = g1 - g0 + 1;
-                __syncthreads();   // the previous group's readers are done
-                // window rows tlo .. tlo + Wp - 1 of each column, whole 256-byte
-                // chunks straight into LDS (padded rows keep them in bounds)
-                {
-                    const int nch = Wp / 32;   // chunks per column
-                    const char* colp = reinterpret_cast<const char*>(pref + (colbase + g0) * gc.cs + tlo) + lane * 4;
-                    const uint64_t cstride = (uint64_t)gc.cs * 8;
-                    for (int c = 0; c < ng; c++) {
-                        const char* src = colp + c * cstride;
-                        char* dst = reinterpret_cast<char*>(Pw) + c * Wp * 8;
-                        for (int i = wave; i < nch; i += kW)
-                            __builtin_amdgcn_global_load_lds(src + i * 256, dst + i * 256, 4, 0, 0);
-                    }
-                    // the group's column records (80 B each)
-                    for (int ch = wave; ch * 64 < ng * 20; ch += kW) {
-                        const char* src = reinterpret_cast<const char*>(gcols + colbase + g0) + ch * 256 + lane * 4;
-                        __builtin_amdgcn_global_load_lds(src, reinterpret_cast<char*>(Cw) + ch * 256, 4, 0, 0);
-                    }
-                    if (g0 == wn.qlo) {
-                        for (int ch = wave; ch * 32 < Wp; ch += kW) {
-                            const char* src =
-                                reinterpret_cast<const char*>(izt + (uint64_t)p * gc.cs + tlo) + ch * 256 + lane * 4;
-                            __builtin_amdgcn_global_load_lds(src, reinterpret_cast<char*>(Iw) + ch * 256, 4, 0, 0);
-                        }
-                        for (int e = tid; e < (kTX + 1) * (kTY + 1); e += kT) {
-                            const int cx = e % (kTX + 1), cy = e / (kTX + 1);
-                            const float dx = (float)min(X0 + cx, rc.hi[0]) - g.sx;
-                            const float dy = (float)min(Y0 + cy, rc.hi[1]) - g.sy;
-                            Qc[e] = -(g.ax * dy - g.ay * dx) * frcp(g.bx * dy - g.by * dx);
-                        }
-                    }
-                    __builtin_amdgcn_s_waitcnt(0);
+        // the batch's items (view, column group) in order, the next one loading
+        // while the current one is worked on
+        uint32_t p = pb;
+        GWin wn = win_at(0);
+        while (p < pe && wn.qlo > wn.qhi) {   // the first view that meets the tile
+            p++;
+            if (p < pe) wn = win_at((int)(p - pb));
+        }
+        if (p >= pe) continue;
+        int g0 = wn.qlo;
+        stage(p, wn, g0, min(wn.qhi, g0 + wn.ncg - 1) - g0 + 1, Buf0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        // one item: stage the next into Bn, work on this one from Bc.  Two named
+        // buffers (not one indexed array) let the compiler see that the work's LDS
+        // reads do not touch the loads in flight, so it does not wait for them.
+        auto step = [&](GBuf& Bc, GBuf& Bn) {
+            const int g1 = min(wn.qhi, g0 + wn.ncg - 1);
+            // the next item: the view's next group, else the next view that meets the tile
+            uint32_t pn = p;
+            int gn = g1 + 1;
+            GWin wnn = wn;
+            if (gn > wn.qhi) {
+                pn = p + 1;
+                if (pn < pe) wnn = win_at((int)(pn - pb));
+                while (pn < pe && wnn.qlo > wnn.qhi) {
+                    pn++;
+                    if (pn < pe) wnn = win_at((int)(pn - pb));
                 }
-                __syncthreads();
-                // ---- setup pass: lane (si, ss) takes column qa(cell si) + ss ----
-                float ha = 0.f, dh = 0.f, aa = 0.f, ba = 0.f, ab = 0.f, bbt = 0.f, lq = 0.f, yhd = 0.f, ifh = 0.f;
-                double gm = 0.0;
-                int pb_ = 0, mp = 0, pcol = 0;
-                bool valid = false;
-                {
-                    const float c00 = Qc[s_ly * (kTX + 1) + s_lx], c10 = Qc[s_ly * (kTX + 1) + s_lx + 1];
-                    const float c01 = Qc[(s_ly + 1) * (kTX + 1) + s_lx], c11 = Qc[(s_ly + 1) * (kTX + 1) + s_lx + 1];
-                    const float cmin = fminf(fminf(c00, c10), fminf(c01, c11));
-                    const float cmax = fmaxf(fmaxf(c00, c10), fmaxf(c01, c11));
-                    const int qa = max(g0, (int)floorf(cmin - 1e-3f));
-                    const int qb = min(g1, (int)ceilf(cmax + 1e-3f));
-                    const int q = qa + ss;
-                    if (s_cv && q <= qb) {
-                        const GCol& G = Cw[q - g0];
-                        float h0 = G.hs, h1 = G.hd;
-                        const int blo[2] = {s_ci, s_cj}, bhi[2] = {s_ci + 1, s_cj + 1};
-#pragma unroll
-                        for (int a = 0; a < 2; a++) {
-                            if (G.sg[a] == 0) {
-                                if (G.ifix[a] != blo[a]) h1 = -AP_BIG;
-                                continue;
-                            }
-                            const float ein = (float)(G.sg[a] > 0 ? blo[a] : bhi[a]) - G.P0[a];
-                            const float eout = (float)(G.sg[a] > 0 ? bhi[a] : blo[a]) - G.P0[a];
-                            h0 = fmaxf(h0, tcross(ein, G.r[a], G.c[a]));
-                            h1 = fminf(h1, tcross(eout, G.r[a], G.c[a]));
-                        }
-                        if (h0 < h1) {
-                            valid = true;
-                            ha = h0;
-                            dh = h1 - h0;
-                            aa = G.lods * frcp(h0 + G.hc);
-                            ab = G.lods * frcp(h1 + G.hc);
-                            ba = __builtin_fmaf(-g.sz, aa, wn.kc);
-                            bbt = __builtin_fmaf(-g.sz, ab, wn.kc);
-                            lq = G.l;
-                            gm = G.gam;
-                            pb_ = (q - g0) * Wp * 8;
-                            pcol = q - g0;
-                            yhd = G.yh * dh;
-                            ifh = (float)G.ifixh;
-                            // rays straddling one plane inside the cell lie in
-                            // (min kappa, max kappa], |kappa_a - kappa_b| = |k - sz| |alpha_a - alpha_b|:
-                            // more than one of them needs the general evaluation
-                            mp = kzspan * fabsf(aa - ab) >= 0.99f ? 1 : 0;
-                        }
-                    }
-                }
-                const char* Pb = reinterpret_cast<const char*>(Pw);
-                // G(k) of one (cell, column) pair at boundary kf (kz = kf - sz in FP64), added to a
-                auto gacc = [&](double& a, float kf, double kz, float ha_, float dh_, float aa_, float ba_, float ab_,
-                                float bb_, float l_, double gm_, int pbb) {
-                    const float ka = __builtin_fmaf(kf, aa_, ba_), kb = __builtin_fmaf(kf, ab_, bb_);
-                    const int tf = cvt_flr(__builtin_amdgcn_fmed3f(fminf(ka, kb), 0.f, Wm2));
-                    const double P0 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8);
-                    const double P1 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8 + 8);
-                    const double iz = Iw[tf];
-                    // crossing of plane k by ray tf, along h - hc, relative to the cell entry
-                    const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
-                    const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
-                    const float F = iz > 0.0 ? Fp : dh_ - Fp;
-                    a = fma((double)dh_, P0, a);
-                    a = fma((double)F, P1 - P0, a);
-                };
-                // the same with every ray that may straddle plane k
-                auto gacc_multi = [&](double& a, float kf, double kz, float ha_, float dh_, float aa_, float ba_,
-                                      float ab_, float bb_, float l_, double gm_, int pbb) {
-                    const float ka = __builtin_fmaf(kf, aa_, ba_), kb = __builtin_fmaf(kf, ab_, bb_);
-                    const int tf = cvt_flr(__builtin_amdgcn_fmed3f(fminf(ka, kb), 0.f, Wm2));
-                    const int te = cvt_flr(__builtin_amdgcn_fmed3f(fmaxf(ka, kb), (float)tf, Wm2));
-                    a = fma((double)dh_, *reinterpret_cast<const double*>(Pb + pbb + tf * 8), a);
-                    for (int t = tf; t <= te; t++) {
-                        const double iz = Iw[t];
-                        const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
-                        const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
-                        const float F = iz > 0.0 ? Fp : dh_ - Fp;
-                        const double P0 = *reinterpret_cast<const double*>(Pb + pbb + t * 8);
-                        const double P1 = *reinterpret_cast<const double*>(Pb + pbb + t * 8 + 8);
-                        a = fma((double)F, P1 - P0, a);
-                    }
-                };
-                {
-                    // flags: bit 24 several rays may straddle a plane, bit 25 the
-                    // column has a ray lying in a z plane, bits 26-29 its column in
-                    // the group (Cw); bits 0-23 the window's byte offset
-                    float* R = Prec[wave][lane];
-                    const uint64_t gbits = (uint64_t)__double_as_longlong(gm);
-                    const int flags = pb_ | (mp << 24) | ((yhd != 0.f ? 1 : 0) << 25) | (pcol << 26);
-                    reinterpret_cast<float4*>(R)[0] = make_float4(ha, dh, aa, ab);
-                    reinterpret_cast<float4*>(R)[1] = make_float4(
-                        lq, __int_as_float(flags), __uint_as_float((uint32_t)gbits), __uint_as_float((uint32_t)(gbits >> 32)));
-                }
-                if (valid) {
-                    double gt = 0.0;
-                    if (mp)
-                        gacc_multi(gt, zf1, kz1d, ha, dh, aa, ba, ab, bbt, lq, gm, pb_);
-                    else
-                        gacc(gt, zf1, kz1d, ha, dh, aa, ba, ab, bbt, lq, gm, pb_);
-                    if (yhd != 0.f && zf1 > ifh) gt += (double)yhd;
-                    topacc += gt;
-                }
-                // ---- main pass: every lane, every pair of cell i ----
-#pragma unroll
-                for (int i = 0; i < kC; i++) {
-                    uint64_t m = __ballot(valid && si == i);
-                    while (m) {
-                        const int src = __builtin_ctzll(m);
-                        m &= m - 1;
-                        // the pair's record, read by every lane at one address (a broadcast)
-                        const float4* R = reinterpret_cast<const float4*>(Prec[wave][src]);
-                        const float4 r0 = R[0], r1 = R[1];
-                        const float rha = r0.x, rdh = r0.y, raa = r0.z, rab = r0.w;
-                        const float rba = __builtin_fmaf(msz, raa, wn.kc), rbb = __builtin_fmaf(msz, rab, wn.kc);
-                        const float rl = r1.x;
-                        const int rfl = __float_as_int(r1.y);
-                        const int rpb = rfl & 0xFFFFFF;
-                        const double rgm = __longlong_as_double(
-                            (long long)(((uint64_t)__float_as_uint(r1.w) << 32) | __float_as_uint(r1.z)));
-                        if (rfl & (1 << 24)) {
-                            for (int jj = 0; jj < kJ; jj++) {
-                                const float kf = (float)(z0 + lane + 64 * jj);
-                                gacc_multi(acc[i][jj], kf, kzd[jj], rha, rdh, raa, rba, rab, rbb, rl, rgm, rpb);
-                            }
-                        } else {
-#pragma unroll
-                            for (int jj = 0; jj < kJ; jj++)
-                                gacc(acc[i][jj], (float)(z0 + lane + 64 * jj), kzd[jj], rha, rdh, raa, rba, rab, rbb,
-                                     rl, rgm, rpb);
-                        }
-                        if (rfl & (1 << 25)) {
-                            // rays lying in a z plane: their voxel row is below the plane
-                            const GCol& G = Cw[(rfl >> 26) & 15];
-                            const double ryh = (double)(G.yh * rdh);
-                            const float rif = (float)G.ifixh;
-#pragma unroll
-                            for (int jj = 0; jj < kJ; jj++)
-                                if ((float)(z0 + lane + 64 * jj) > rif) acc[i][jj] += ryh;
-                        }
-                    }
-                }
+                gn = wnn.qlo;
             }
+            if (pn < pe) stage(pn, wnn, gn, min(wnn.qhi, gn + wnn.ncg - 1) - gn + 1, Bn);
+            work(p, wn, g0, g1, Bc);
+            __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of the next item have landed
+            __syncthreads();                 // all have, and the current buffer's readers are done
+            p = pn;
+            g0 = gn;
+            wn = wnn;
+        };
+        while (p < pe) {
+            step(Buf0, Buf1);
+            if (p >= pe) break;
+            step(Buf1, Buf0);
         }
     }
     // ---- G at z1 per cell: sum over the lanes of the cell's setup slots ----
@@ -588,7 +749,7 @@ __device__ bool gather_view_ok(const RayCfg& rc, const ap_proj_spec* __restrict_
 
 namespace {
 
-constexpr int kGW = 8, kGCv = 4, kGJ = 4, kGTX = 8;
+constexpr int kGW = 16, kGCv = 4, kGJ = 4, kGTX = 8;
 
 }  // namespace
 
@@ -641,7 +802,9 @@ int32_t launch_gather_adjoint(const RayCfg& rc, uint32_t V, uint32_t nu, uint32_
     tl.ntx = (uint32_t)(rc.hi[0] - rc.lo[0] + kGTX - 1) / kGTX;
     tl.nty = (uint32_t)(rc.hi[1] - rc.lo[1] + kTY - 1) / kTY;
     tl.ntz = (uint32_t)(rc.hi[2] - rc.lo[2] + 64 * kGJ - 1) / (64 * kGJ);
-    const uint64_t total = (uint64_t)tl.ntx * tl.nty * tl.ntz;
+    tl.nsx = (tl.ntx + kSTX - 1) / kSTX;
+    tl.nsy = (tl.nty + kSTY - 1) / kSTY;
+    const uint64_t total = (uint64_t)tl.nsx * tl.nsy * kST * tl.ntz;
     if (total >= (1ull << 31)) return ap_fail(AP_ERR_INVALID, "too many tiles");
     tl.total = (uint32_t)total;
     tl.per_xcd = (tl.total + kXcdG - 1) / kXcdG;

@@ -48,18 +48,19 @@ def _adjoints(pg, n, w, y, cfg):
 
 def test_noalloc_matches_workspace_entry_point(device):
     rng = np.random.default_rng(21)
-    # 1) 2048 x 1000 detector: one view needs ~17 MB of prefix sums, so the
-    #    64 MiB scratch takes the 8 views 3 at a time
+    # 1) 2048 x 1000 detector: one view needs ~26 MB of prefix sums, so the
+    #    64 MiB scratch takes the 8 views in several passes
     pg = ph.circular_geometry(8, 2048, 1000, 0.05, R=60.0, L=120.0)
     n, w = 32, 0.5
     y = rng.uniform(0, 1, (8, 2048, 1000)).astype(np.float32)
     cfg = cbfp.ConfigProject()
     per_view = _abi.lib().ap_backproject_matched_workspace_size(_abi.struct_ptr(cfg.contents), 1,
                                                                 _abi.array3(__import__("torch").zeros((1, 2048, 1000))))
-    assert 3 * per_view <= 64 << 20 < 4 * per_view
+    per_pass = (64 << 20) // per_view
+    assert 2 <= per_pass < 8
     got, want = _adjoints(pg, n, w, y, cfg)
     err = rel_l2(got, want)
-    print(f"noalloc vs workspace, 8 views of 2048x1000 (3 per scratch pass): rel L2 {err:.3e}")
+    print(f"noalloc vs workspace, 8 views of 2048x1000 ({per_pass} per scratch pass): rel L2 {err:.3e}")
     assert np.abs(want).max() > 0 and err <= 1e-6
     # 2) a second, smaller call on the same scratch: one pass; also against the oracle
     pg2 = ph.circular_geometry(6, 40, 36, 1.0, R=60.0, L=120.0)
