@@ -23,9 +23,10 @@ read for --calibrate-copy; it is null when that file is absent.  With it,
 roofline.measured_gbs = traffic / kernel time.  roofline.valu_issue_frac is
 SQ_INSTS_VALU per launch (profiles/sq_instruction_mix_siddon_512cube_720views.md)
 over the VALU issue capacity in the kernel's time (one wave64 VALU
-instruction per CU per cycle, 256 CUs at 2.4 GHz), and roofline.bound names
-the larger of the two fractions ("valu" or "hbm"); with no counter file the
-bound stays the per-view model's "hbm".
+instruction per CU per cycle, 256 CUs at 2.4 GHz), roofline.lds_active_frac
+the LDS-active cycles (SQ_LDS_IDX_ACTIVE) over the same cycles, and
+roofline.bound names the largest of the three fractions ("hbm", "valu" or
+"lds"); with no counter file the bound stays the per-view model's "hbm".
 """
 
 from __future__ import annotations
@@ -453,6 +454,13 @@ def read_valu(kernel: str):
     return None if v is None else v[0]
 
 
+def read_lds_active(kernel: str):
+    """SQ_LDS_IDX_ACTIVE per launch of `kernel` (LDS-active cycles summed over
+    the CUs) from the committed rocprofv3 summary."""
+    v = _summary_line(SQ_FILE, "sq_lds_idx_active_per_launch", kernel, 1)
+    return None if v is None else v[0]
+
+
 def calibrate_copy() -> int:
     """Two plain device-to-device copies of 1 GiB each (2 GiB read, 2 GiB
     written), the reference for scaling FETCH_SIZE / WRITE_SIZE."""
@@ -560,11 +568,14 @@ def main() -> int:
     achieved = b_op / t_dom / 1e9
     traffic = read_traffic(dom_kernel) if args.volume_dtype == "f32" else None
     valu = read_valu(dom_kernel) if args.volume_dtype == "f32" else None
+    lds = read_lds_active(dom_kernel) if args.volume_dtype == "f32" else None
     measured_gbs = None if traffic is None else traffic / t_dom / 1e9
     valu_frac = None if valu is None else valu / (VALU_ISSUE_PER_S * t_dom)
-    bound = "hbm"
-    if valu_frac is not None and measured_gbs is not None and valu_frac > measured_gbs / HBM_PEAK_GBS:
-        bound = "valu"
+    # LDS-active cycles per CU over the cycles of the kernel's time
+    lds_frac = None if lds is None else lds / (VALU_ISSUE_PER_S * t_dom)
+    fracs = {"hbm": None if measured_gbs is None else measured_gbs / HBM_PEAK_GBS, "valu": valu_frac, "lds": lds_frac}
+    known = {k: v for k, v in fracs.items() if v is not None}
+    bound = max(known, key=known.get) if len(known) > 1 else "hbm"
     out = {
         "metric": METRIC,
         "value": gups,
@@ -592,11 +603,13 @@ def main() -> int:
                      "measured_gbs": measured_gbs,
                      "measured_frac": None if measured_gbs is None else measured_gbs / HBM_PEAK_GBS,
                      "valu_issue_frac": valu_frac,
+                     "lds_active_frac": lds_frac,
                      "model": "achieved = B_op / t with B_op = V (N^3 s_v + nu nv s_p) bytes per operator "
                               "application and t = the operator call on its stream (HIP events); "
                               "measured_gbs = rocprofv3 FETCH+WRITE bytes per launch / t; valu_issue_frac = "
-                              "SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x t); bound = the larger of "
-                              "measured_frac and valu_issue_frac"},
+                              "SQ_INSTS_VALU per launch / (256 CUs x 2.4 GHz x t); lds_active_frac = "
+                              "SQ_LDS_IDX_ACTIVE per launch / (256 CUs x 2.4 GHz x t); bound = the largest of "
+                              "measured_frac, valu_issue_frac and lds_active_frac"},
     }
     if args.volume_dtype == "f32" and not args.no_storage:
         log("16-bit storage leg")
