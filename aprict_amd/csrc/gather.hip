@@ -37,9 +37,10 @@
 // works on one item, the next one's prefix window is loaded straight into
 // the other of two LDS buffers (16-byte loads).  Per item each wave derives
 // its (cell, column) pairs one per lane, then for every pair every lane
-// evaluates G at its boundaries (14 VALU operations and two LDS reads on
-// tiles that lie on one side of the source plane, 18 otherwise) and adds it
-// to its FP64 accumulators.  The voxel values are differences of
+// evaluates G at its boundaries (10 VALU operations and two LDS reads on
+// tiles that lie on one side of the source plane, 16 otherwise) and adds it
+// to its FP64 accumulators.  The part of ray tf below plane k is one FP64 FMA
+// in units of the cell's chord, clamped to [0, 1] by the FMA's clamp modifier.  The voxel values are differences of
 // neighbouring boundaries, scaled and added to the volume once at the end.
 // Tiles run in blocks of 4x8, so that the workgroups on one XCD at one time
 // share their views' prefix windows through its L2.
@@ -47,8 +48,8 @@
 // Precision: the sums P and the accumulators are FP64, so a voxel's value is
 // exact to the FP32 rounding of its own terms whatever the dynamic range of
 // the sinogram around it; the crossing of plane k by ray tf is formed in FP64
-// relative to the column's closest approach (FP32 error of the forward's own
-// crossing parameters).  A ray lying in a z plane (the forward's horizontal
+// from the column's closest approach, with l / dh rounded to FP32 (the FP32
+// error of the forward's own crossing parameters).  A ray lying in a z plane (the forward's horizontal
 // rows, |slope| < 1e-20) belongs to the voxel below the plane, as in the
 // forward; such rays are kept out of P and added separately.
 #include <mutex>
@@ -253,6 +254,13 @@ struct GWin {
     int pad[2];
 };
 
+// a view's parameters, as wave-uniform values
+struct GVu {
+    float sz, sx, sy;
+    int t0;
+    float ax, ay, bx, by;
+};
+
 template <typename Tv, typename Tm, int kW, int kC, int kJ, int kTX>
 __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
     RayCfg rc, GCfg gc, GTile tl, const GView* __restrict__ gviews, const GCol* __restrict__ gcols,
@@ -276,8 +284,12 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
     __shared__ __align__(16) GBuf Buf0;
     __shared__ __align__(16) GBuf Buf1;
     __shared__ GWin Win[64];
+    // the batch's view parameters, one 32-byte record per view: sz, sx, sy, t0 (bits),
+    // ax, ay, bx, by -- read from LDS at the start of a step, so that no step waits
+    // on a global load of its view record
+    __shared__ __align__(16) float Vw[64 * 8];
     // per-wave (cell, column) pair records of the setup pass, one per lane:
-    // h_a, h_b - h_a, alpha_a, alpha_b, l, window offset | flags, hc / l (FP64).
+    // l / dh, dh = h_b - h_a, alpha_a, alpha_b, 0, window offset | flags, H (FP64).
     // The records start 8 bytes past a 16-byte boundary: the compiler then reads
     // them as 8-byte pairs, which keep its knowledge that they do not touch the
     // window loads in flight (its 16-byte reads lose it and wait for them)
@@ -325,11 +337,10 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
     // load item (view p, columns g0 .. g0 + ng - 1) into buffer B: 1/dz and P
     // rows in 1 KB pieces (16 bytes per lane, straight into LDS), the last piece
     // of each row set with only the lanes it needs
-    auto stage = [&](uint32_t p, const GWin& wn, int g0, int ng, GBuf& B) {
+    auto stage = [&](uint32_t p, const GWin& wn, const GVu& g, int g0, int ng, GBuf& B) {
         // the cell corners' projections (column coordinate), written before the
         // loads into LDS are issued (no wait on them)
         if (tid < kNQc) {
-            const GView g = gviews[p];
             const int cx = tid % (kTX + 1), cy = tid / (kTX + 1);
             const float dx = (float)min(X0 + cx, rc.hi[0]) - g.sx;
             const float dy = (float)min(Y0 + cy, rc.hi[1]) - g.sy;
@@ -361,8 +372,7 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
     };
 
     // the setup and main passes of item (p, g0 .. g1) from buffer B
-    auto work = [&](uint32_t p, const GWin& wn, int g0, int g1, const GBuf& B) {
-        const GView g = gviews[p];
+    auto work = [&](const GWin& wn, const GVu& g, int g0, int g1, const GBuf& B) {
         const int W = wn.W, tlo = wn.tlo;
         const int Wp = (W + 31) & ~31;
         // k - sz per slot, exact in FP64 (k an integer below 2^24, sz a float)
@@ -388,8 +398,8 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
         const char* Pb = reinterpret_cast<const char*>(B.win);   // 1/dz rows at 0, column c at (c + 1) Wp 8
         const double* Iw = B.win;
         // ---- setup pass: lane (si, ss) takes column qa(cell si) + ss ----
-        float ha = 0.f, dh = 0.f, aa = 0.f, ba = 0.f, ab = 0.f, bbt = 0.f, lq = 0.f, yhd = 0.f, ifh = 0.f;
-        double gm = 0.0;
+        float dh = 0.f, aa = 0.f, ba = 0.f, ab = 0.f, bbt = 0.f, rlq = 0.f, yhd = 0.f, ifh = 0.f;
+        double hq = 0.0;
         int pb_ = 0, mp = 0, pcol = 0;
         bool valid = false;
         {
@@ -401,34 +411,63 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
             const int qb = min(g1, (int)ceilf(cmax + 1e-3f));
             const int q = qa + ss;
             if (s_cv && q <= qb) {
-                const GCol& G = B.cols[q - g0];
-                float h0 = G.hs, h1 = G.hd;
+                // the column record, all 80 bytes in one block of reads with one wait
+                // (read field by field, its reads sit in the branches below, each with
+                // its own wait)
+                const uint32_t ca =
+                    (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const char*)(const char*)&B.cols[q - g0];
+                float4 a0, a1, a2, a3, a4;
+                asm volatile(
+                    "ds_read_b128 %0, %5\n\t"
+                    "ds_read_b128 %1, %5 offset:16\n\t"
+                    "ds_read_b128 %2, %5 offset:32\n\t"
+                    "ds_read_b128 %3, %5 offset:48\n\t"
+                    "ds_read_b128 %4, %5 offset:64\n\t"
+                    "s_waitcnt lgkmcnt(0)"
+                    : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(a4)   // none shares the address register
+                    : "v"(ca)
+                    : "memory");
+                // GCol: r[2] c[2] | P0[2] sg[2] | ifix[2] hs hd | l hc gam | yh ifixh lods pad
+                const float Gr[2] = {a0.x, a0.y}, Gc[2] = {a0.z, a0.w}, GP0[2] = {a1.x, a1.y};
+                const int Gsg[2] = {__float_as_int(a1.z), __float_as_int(a1.w)};
+                const int Gif[2] = {__float_as_int(a2.x), __float_as_int(a2.y)};
+                float h0 = a2.z, h1 = a2.w;
+                const float Gl = a3.x, Ghc = a3.y;
+                const double Ggam =
+                    __longlong_as_double((long long)(((uint64_t)__float_as_uint(a3.w) << 32) | __float_as_uint(a3.z)));
+                const float Gyh = a4.x, Glods = a4.z;
+                const int Gifh = __float_as_int(a4.y);
                 const int blo[2] = {s_ci, s_cj}, bhi[2] = {s_ci + 1, s_cj + 1};
 #pragma unroll
                 for (int a = 0; a < 2; a++) {
-                    if (G.sg[a] == 0) {
-                        if (G.ifix[a] != blo[a]) h1 = -AP_BIG;
+                    if (Gsg[a] == 0) {
+                        if (Gif[a] != blo[a]) h1 = -AP_BIG;
                         continue;
                     }
-                    const float ein = (float)(G.sg[a] > 0 ? blo[a] : bhi[a]) - G.P0[a];
-                    const float eout = (float)(G.sg[a] > 0 ? bhi[a] : blo[a]) - G.P0[a];
-                    h0 = fmaxf(h0, tcross(ein, G.r[a], G.c[a]));
-                    h1 = fminf(h1, tcross(eout, G.r[a], G.c[a]));
+                    const float ein = (float)(Gsg[a] > 0 ? blo[a] : bhi[a]) - GP0[a];
+                    const float eout = (float)(Gsg[a] > 0 ? bhi[a] : blo[a]) - GP0[a];
+                    h0 = fmaxf(h0, tcross(ein, Gr[a], Gc[a]));
+                    h1 = fminf(h1, tcross(eout, Gr[a], Gc[a]));
                 }
                 if (h0 < h1) {
                     valid = true;
-                    ha = h0;
                     dh = h1 - h0;
-                    aa = G.lods * frcp(h0 + G.hc);
-                    ab = G.lods * frcp(h1 + G.hc);
+                    aa = Glods * frcp(h0 + Ghc);
+                    ab = Glods * frcp(h1 + Ghc);
                     ba = __builtin_fmaf(-g.sz, aa, wn.kc);
                     bbt = __builtin_fmaf(-g.sz, ab, wn.kc);
-                    lq = G.l;
-                    gm = G.gam;
+                    // the crossing in units of the chord: x' = (k - sz) / dz_t * l / dh - H,
+                    // H = (hc / l) (l / dh) + h_a / dh.  A chord too short to divide by
+                    // keeps x' = 0, an error of at most dh yhat_tf.
+                    if (dh > 1e-20f) {
+                        const double rd = 1.0 / (double)dh;
+                        rlq = (float)((double)Gl * rd);
+                        hq = fma(Ggam, (double)rlq, (double)h0 * rd);
+                    }
                     pb_ = (q - g0 + 1) * Wp * 8;
                     pcol = q - g0;
-                    yhd = G.yh * dh;
-                    ifh = (float)G.ifixh;
+                    yhd = Gyh * dh;
+                    ifh = (float)Gifh;
                     // rays straddling one plane inside the cell lie in
                     // (min kappa, max kappa], |kappa_a - kappa_b| = |k - sz| |alpha_a - alpha_b|:
                     // more than one of them needs the general evaluation
@@ -436,80 +475,78 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
                 }
             }
         }
-        // G(k) of one (cell, column) pair at boundary kf (kz = kf - sz in FP64), added to a
-        auto gacc = [&](double& a, float kf, double kz, float ha_, float dh_, float aa_, float ba_, float ab_,
-                        float bb_, float l_, double gm_, int pbb) {
+        // The part of ray t's chord below plane k, in units of the chord:
+        // clamp((k - sz) / dz_t * l / dh - H, 0, 1) for a rising ray (one FP64 FMA with
+        // the clamp modifier), one minus that for a falling one.
+        auto below = [](double kz, double iz, float rl_, double h_) -> double {
+            return __builtin_fmin(__builtin_fmax(fma(kz * iz, (double)rl_, -h_), 0.0), 1.0);
+        };
+        // G(k) of one (cell, column) pair at boundary kf (kz = kf - sz in FP64), added to a:
+        // dh (P[tf] + F' (P[tf + 1] - P[tf]))
+        auto gacc = [&](double& a, float kf, double kz, float dh_, float aa_, float ba_, float ab_, float bb_,
+                        float rl_, double h_, int pbb) {
             const float ka = __builtin_fmaf(kf, aa_, ba_), kb = __builtin_fmaf(kf, ab_, bb_);
             const int tf = cvt_flr(__builtin_amdgcn_fmed3f(fminf(ka, kb), 0.f, Wm2));
             const double P0 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8);
             const double P1 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8 + 8);
             const double iz = Iw[tf];
-            // crossing of plane k by ray tf, along h - hc, relative to the cell entry
-            const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
-            const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
-            const float F = iz > 0.0 ? Fp : dh_ - Fp;
-            a = fma((double)dh_, P0, a);
-            a = fma((double)F, P1 - P0, a);
+            const double x = below(kz, iz, rl_, h_);
+            const double F = iz > 0.0 ? x : 1.0 - x;
+            a = fma((double)dh_, fma(F, P1 - P0, P0), a);
         };
         // the same with every ray that may straddle plane k
-        auto gacc_multi = [&](double& a, float kf, double kz, float ha_, float dh_, float aa_, float ba_,
-                              float ab_, float bb_, float l_, double gm_, int pbb) {
+        auto gacc_multi = [&](double& a, float kf, double kz, float dh_, float aa_, float ba_, float ab_, float bb_,
+                              float rl_, double h_, int pbb) {
             const float ka = __builtin_fmaf(kf, aa_, ba_), kb = __builtin_fmaf(kf, ab_, bb_);
             const int tf = cvt_flr(__builtin_amdgcn_fmed3f(fminf(ka, kb), 0.f, Wm2));
             const int te = cvt_flr(__builtin_amdgcn_fmed3f(fmaxf(ka, kb), (float)tf, Wm2));
-            a = fma((double)dh_, *reinterpret_cast<const double*>(Pb + pbb + tf * 8), a);
+            const double d = (double)dh_;
+            a = fma(d, *reinterpret_cast<const double*>(Pb + pbb + tf * 8), a);
             for (int t = tf; t <= te; t++) {
                 const double iz = Iw[t];
-                const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
-                const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
-                const float F = iz > 0.0 ? Fp : dh_ - Fp;
+                const double x = below(kz, iz, rl_, h_);
+                const double F = iz > 0.0 ? x : 1.0 - x;
                 const double P0 = *reinterpret_cast<const double*>(Pb + pbb + t * 8);
                 const double P1 = *reinterpret_cast<const double*>(Pb + pbb + t * 8 + 8);
-                a = fma((double)F, P1 - P0, a);
+                a = fma(d * F, P1 - P0, a);
             }
         };
         // G(k) on one side of the source (kap: kappa at the end that is the smaller
-        // at every boundary of the tile); rising rays: dh P[tf] + F (P[tf+1] - P[tf]),
-        // falling rays: dh P[tf+1] - Fp (P[tf+1] - P[tf]) with F = dh - Fp
-        auto gup = [&](double& a, float kap, double kz, float ha_, float dh_, float l_, double gm_, int pbb) {
+        // at every boundary of the tile); rising rays: dh (P[tf] + F' (P[tf+1] - P[tf])),
+        // falling rays: dh (P[tf+1] - x' (P[tf+1] - P[tf])) with F' = 1 - x'
+        auto gup = [&](double& a, float kap, double kz, double dh_, float rl_, double h_, int pbb) {
             const int tf = cvt_flr(__builtin_amdgcn_fmed3f(kap, tlo_up, Wm2));
             const double P0 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8);
             const double P1 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8 + 8);
             const double iz = Iw[tf];
-            const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
-            const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
-            a = fma((double)dh_, P0, a);
-            a = fma((double)Fp, P1 - P0, a);
+            a = fma(dh_, fma(below(kz, iz, rl_, h_), P1 - P0, P0), a);
         };
-        auto gdn = [&](double& a, float kap, double kz, float ha_, float dh_, float l_, double gm_, int pbb) {
+        auto gdn = [&](double& a, float kap, double kz, double dh_, float rl_, double h_, int pbb) {
             const int tf = cvt_flr(__builtin_amdgcn_fmed3f(kap, 0.f, thi_dn));
             const double P0 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8);
             const double P1 = *reinterpret_cast<const double*>(Pb + pbb + tf * 8 + 8);
             const double iz = Iw[tf];
-            const float x = __builtin_fmaf((float)fma(kz, iz, -gm_), l_, -ha_);
-            const float Fp = __builtin_amdgcn_fmed3f(x, 0.f, dh_);
-            a = fma((double)dh_, P1, a);
-            a = fma(-(double)Fp, P1 - P0, a);
+            a = fma(dh_, fma(-below(kz, iz, rl_, h_), P1 - P0, P1), a);
         };
         {
             // flags: bit 24 several rays may straddle a plane, bit 25 the
             // column has a ray lying in a z plane, bits 26-29 its column in
             // the group; bits 0-23 the byte offset of its rows in the window
             float* R = prec(wave, lane);
-            const uint64_t gbits = (uint64_t)__double_as_longlong(gm);
+            const uint64_t gbits = (uint64_t)__double_as_longlong(hq);
             const int flags = pb_ | (mp << 24) | ((yhd != 0.f ? 1 : 0) << 25) | (pcol << 26);
-            reinterpret_cast<float2*>(R)[0] = make_float2(ha, dh);
+            reinterpret_cast<float2*>(R)[0] = make_float2(rlq, dh);
             reinterpret_cast<float2*>(R)[1] = make_float2(aa, ab);
-            reinterpret_cast<float2*>(R)[2] = make_float2(lq, __int_as_float(flags));
+            reinterpret_cast<float2*>(R)[2] = make_float2(0.f, __int_as_float(flags));
             reinterpret_cast<float2*>(R)[3] =
                 make_float2(__uint_as_float((uint32_t)gbits), __uint_as_float((uint32_t)(gbits >> 32)));
         }
         if (valid) {
             double gt = 0.0;
             if (mp)
-                gacc_multi(gt, zf1, kz1d, ha, dh, aa, ba, ab, bbt, lq, gm, pb_);
+                gacc_multi(gt, zf1, kz1d, dh, aa, ba, ab, bbt, rlq, hq, pb_);
             else
-                gacc(gt, zf1, kz1d, ha, dh, aa, ba, ab, bbt, lq, gm, pb_);
+                gacc(gt, zf1, kz1d, dh, aa, ba, ab, bbt, rlq, hq, pb_);
             if (yhd != 0.f && zf1 > ifh) gt += (double)yhd;
             topacc += gt;
         }
@@ -533,17 +570,17 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
                     : "=&v"(q0), "=&v"(q1)   // early clobber: no output shares the address register
                     : "v"(ra)
                     : "memory");
-                const float rha = q0.x, rdh = q0.y, raa = q0.z, rab = q0.w;
-                const float rl = q1.x;
+                const float rl = q0.x, rdh = q0.y, raa = q0.z, rab = q0.w;
+                const double rdhd = (double)rdh;
                 const int rfl = __float_as_int(q1.y);
                 const int rpb = rfl & 0xFFFFFF;
-                const double rgm = __longlong_as_double(
+                const double rh = __longlong_as_double(
                     (long long)(((uint64_t)__float_as_uint(q1.w) << 32) | __float_as_uint(q1.z)));
                 if (rfl & (1 << 24)) {
                     const float rba = __builtin_fmaf(msz, raa, wn.kc), rbb = __builtin_fmaf(msz, rab, wn.kc);
                     for (int jj = 0; jj < kJ; jj++) {
                         const float kf = (float)(z0 + lane + 64 * jj);
-                        gacc_multi(acc[i][jj], kf, kzd[jj], rha, rdh, raa, rba, rab, rbb, rl, rgm, rpb);
+                        gacc_multi(acc[i][jj], kf, kzd[jj], rdh, raa, rba, rab, rbb, rl, rh, rpb);
                     }
                 } else if (upv) {
                     // kappa_a - kappa_b has the sign of alpha_a - alpha_b
@@ -551,22 +588,21 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
                     const float rbm = __builtin_fmaf(msz, ram, wn.kc);
 #pragma unroll
                     for (int jj = 0; jj < kJ; jj++)
-                        gup(acc[i][jj], __builtin_fmaf((float)(z0 + lane + 64 * jj), ram, rbm), kzd[jj], rha, rdh, rl,
-                            rgm, rpb);
+                        gup(acc[i][jj], __builtin_fmaf((float)(z0 + lane + 64 * jj), ram, rbm), kzd[jj], rdhd, rl,
+                            rh, rpb);
                 } else if (dnv) {
                     // kappa_a - kappa_b has the sign of alpha_b - alpha_a
                     const float ram = raa >= rab ? raa : rab;
                     const float rbm = __builtin_fmaf(msz, ram, wn.kc);
 #pragma unroll
                     for (int jj = 0; jj < kJ; jj++)
-                        gdn(acc[i][jj], __builtin_fmaf((float)(z0 + lane + 64 * jj), ram, rbm), kzd[jj], rha, rdh, rl,
-                            rgm, rpb);
+                        gdn(acc[i][jj], __builtin_fmaf((float)(z0 + lane + 64 * jj), ram, rbm), kzd[jj], rdhd, rl,
+                            rh, rpb);
                 } else {
                     const float rba = __builtin_fmaf(msz, raa, wn.kc), rbb = __builtin_fmaf(msz, rab, wn.kc);
 #pragma unroll
                     for (int jj = 0; jj < kJ; jj++)
-                        gacc(acc[i][jj], (float)(z0 + lane + 64 * jj), kzd[jj], rha, rdh, raa, rba, rab, rbb, rl, rgm,
-                             rpb);
+                        gacc(acc[i][jj], (float)(z0 + lane + 64 * jj), kzd[jj], rdh, raa, rba, rab, rbb, rl, rh, rpb);
                 }
                 if (rfl & (1 << 25)) {
                     // rays lying in a z plane: their voxel row is below the plane
@@ -597,15 +633,42 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
         return u;
     };
 
+    // view record idx of the batch, read as wave-uniform values
+    auto vw_at = [&](int idx) -> GVu {
+        const uint32_t va = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)(Vw + idx * 8);
+        float4 v0, v1;
+        asm volatile(
+            "ds_read2_b64 %0, %2 offset1:1\n\t"
+            "ds_read2_b64 %1, %2 offset0:2 offset1:3\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(v0), "=&v"(v1)   // early clobber: no output shares the address register
+            : "v"(va)
+            : "memory");
+        auto u = [](float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); };
+        GVu g;
+        g.sz = u(v0.x);
+        g.sx = u(v0.y);
+        g.sy = u(v0.z);
+        g.t0 = __builtin_amdgcn_readfirstlane(__float_as_int(v0.w));
+        g.ax = u(v1.x);
+        g.ay = u(v1.y);
+        g.bx = u(v1.z);
+        g.by = u(v1.w);
+        return g;
+    };
+
     for (uint32_t pb = 0; pb < gc.V; pb += 64) {
-        // ---- windows of views pb .. pb + 63, one per lane of wave 0 ----
-        __syncthreads();   // the previous batch's readers of Win are done
+        // ---- windows and view records of views pb .. pb + 63, one per lane of wave 0 ----
+        __syncthreads();   // the previous batch's readers of Win and Vw are done
         if (wave == 0) {
             GWin wn;
             wn.qlo = 0; wn.qhi = -1; wn.tlo = 0; wn.W = 2; wn.kc = 0.f; wn.ncg = 1; wn.pad[0] = wn.pad[1] = 0;
             const uint32_t p = pb + (uint32_t)lane;
+            float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
             if (p < gc.V && gviews[p].ok) {
                 const GView g = gviews[p];
+                v0 = make_float4(g.sz, g.sx, g.sy, __int_as_float(g.t0));
+                v1 = make_float4(g.ax, g.ay, g.bx, g.by);
                 float qmin = AP_BIG, qmax = -AP_BIG, dmax = 0.f;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
@@ -652,6 +715,8 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
                 }
             }
             Win[lane] = wn;
+            reinterpret_cast<float4*>(Vw)[lane * 2] = v0;
+            reinterpret_cast<float4*>(Vw)[lane * 2 + 1] = v1;
         }
         __syncthreads();
         const uint32_t pe = min(gc.V, pb + 64);
@@ -665,7 +730,8 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
         }
         if (p >= pe) continue;
         int g0 = wn.qlo;
-        stage(p, wn, g0, min(wn.qhi, g0 + wn.ncg - 1) - g0 + 1, Buf0);
+        GVu vw = vw_at((int)(p - pb));
+        stage(p, wn, vw, g0, min(wn.qhi, g0 + wn.ncg - 1) - g0 + 1, Buf0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         // one item: stage the next into Bn, work on this one from Bc.  Two named
@@ -686,13 +752,16 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
                 }
                 gn = wnn.qlo;
             }
-            if (pn < pe) stage(pn, wnn, gn, min(wnn.qhi, gn + wnn.ncg - 1) - gn + 1, Bn);
-            work(p, wn, g0, g1, Bc);
+            GVu vwn = vw;
+            if (pn != p && pn < pe) vwn = vw_at((int)(pn - pb));
+            if (pn < pe) stage(pn, wnn, vwn, gn, min(wnn.qhi, gn + wnn.ncg - 1) - gn + 1, Bn);
+            work(wn, vw, g0, g1, Bc);
             __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of the next item have landed
             __syncthreads();                 // all have, and the current buffer's readers are done
             p = pn;
             g0 = gn;
             wn = wnn;
+            vw = vwn;
         };
         while (p < pe) {
             step(Buf0, Buf1);
