@@ -426,6 +426,120 @@ __device__ __forceinline__ void bp_fast_views4(const BpCfg& c, const ViewMap* m,
     }
 }
 
+// A view is z-coherent when its projective map does not depend on z except
+// through cv: W.z = 0 and Au.z = 0 (detector normal and u axis horizontal),
+// and, where the Z weights or a weighting preset apply, Zc.z = 0 (horizontal
+// source->detector axis).  Along a voxel column the denominator, cu, the
+// u taps and the weight are then constant and cv is linear in z.
+__device__ __forceinline__ bool view_z_coherent(const BpCfg& c, const ViewMap& m) {
+    const bool zdep = c.zl != 0 || c.zv != 0 || (c.direct && c.preset != 0);
+    return m.W[2] == 0.f && m.Au[2] == 0.f && (!zdep || m.Zc[2] == 0.f);
+}
+
+// The interpolating branch for one z-coherent view at kZ voxels of one
+// column (x, y) at heights z[m]: the u part -- denominator, cu, the weight,
+// the u taps -- once, then per voxel cv = cv(0) + z dcv, its v taps and the
+// blend of bp_fast_view / sample_det.  Reads use clamped in-range addresses
+// and are zeroed where sample_det reads nothing.  Needs 32-bit detector
+// offsets, and for bilinear taps the paired layout (c.pair).
+template <int kZ, typename Tp>
+__device__ __forceinline__ void bp_zcol_view(const BpCfg& c, const ViewMap& M, const Tp* __restrict__ img,
+                                             float x0, float x1, const float (&z)[kZ], const bool (&act)[kZ],
+                                             float (&acc)[kZ]) {
+    const float wx = M.W[0] * x0 + M.W[1] * x1 + M.w0;
+    const float iw = __builtin_amdgcn_rcpf(wx);
+    const float cu = (M.Au[0] * x0 + M.Au[1] * x1 + M.au) * iw;
+    const float cv0 = (M.Av[0] * x0 + M.Av[1] * x1 + M.av) * iw;
+    const float dcv = M.Av[2] * iw;
+    float mult = c.direct ? 1.f : M.kq * c.w3 * (iw * iw);
+    const bool pre = c.direct && c.preset != 0;
+    if (c.zl != 0 || c.zv != 0 || pre) {
+        const float zz = M.Zc[0] * x0 + M.Zc[1] * x1 + M.z0;
+        if (c.zl != 0) mult *= powi(zz * M.inv_Lnorm, c.zl);
+        if (c.zv != 0) mult *= powi(zz * c.inv_zval, c.zv);
+        if (pre) {
+            const float xp[3] = {x0, x1, 0.f};   // the presets use x and y only
+            mult = apply_preset(c, M, xp, zz, mult);
+        }
+    }
+    const float fj = cu + M.hu;
+    float cv[kZ], y[kZ];
+    if (c.bilinear) {
+        const bool uin = fj > -1.f && fj < (float)c.nu;
+        const float fjc = uin ? fj : 0.f;   // finite before the conversion
+        const float fl_j = floorf(fjc);
+        const int j0 = (int)fl_j;
+        const float a = fjc - fl_j;
+        const int ja = max(j0, 0), jb = min(j0 + 1, (int)c.nu - 1);
+        const int ca = ja * (int32_t)c.s1, cb = jb * (int32_t)c.s1;
+        const bool jlo = j0 < 0, jhi = j0 + 1 >= (int)c.nu;
+        TapPair<Tp> pa[kZ], pb[kZ];
+        float b[kZ];
+        bool inr[kZ], klo[kZ], khi[kZ];
+        bool edge = false;
+#pragma unroll
+        for (int m = 0; m < kZ; m++) {
+            cv[m] = __builtin_fmaf(z[m], dcv, cv0);
+            const float fk = cv[m] + M.hv;
+            inr[m] = uin && fk > -1.f && fk < (float)c.nv;
+            const float fkc = inr[m] ? fk : 0.f;
+            const float fl_k = floorf(fkc);
+            const int k0 = (int)fl_k;
+            b[m] = fkc - fl_k;
+            const int kc = min(max(k0, 0), (int)c.nv - 2);
+            pa[m] = *reinterpret_cast<const TapPair<Tp>*>(img + (ca + kc));
+            pb[m] = *reinterpret_cast<const TapPair<Tp>*>(img + (cb + kc));
+            klo[m] = k0 < 0;
+            khi[m] = k0 + 1 >= (int)c.nv;
+            edge = edge || (act[m] && (!inr[m] || klo[m] || khi[m] || jlo || jhi));
+        }
+        if (__ballot(edge) == 0) {
+            // no tap of the wave's active voxels falls off the detector
+#pragma unroll
+            for (int m = 0; m < kZ; m++) {
+                const float t1 = to_f32(pa[m].lo) * (1.f - a) + to_f32(pb[m].lo) * a;
+                const float t2 = to_f32(pa[m].hi) * (1.f - a) + to_f32(pb[m].hi) * a;
+                y[m] = t1 * (1.f - b[m]) + t2 * b[m];
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < kZ; m++) {
+                y[m] = 0.f;
+                if (inr[m]) {
+                    float y00 = klo[m] ? 0.f : (khi[m] ? to_f32(pa[m].hi) : to_f32(pa[m].lo));
+                    float y01 = khi[m] ? 0.f : (klo[m] ? to_f32(pa[m].lo) : to_f32(pa[m].hi));
+                    float y10 = klo[m] ? 0.f : (khi[m] ? to_f32(pb[m].hi) : to_f32(pb[m].lo));
+                    float y11 = khi[m] ? 0.f : (klo[m] ? to_f32(pb[m].lo) : to_f32(pb[m].hi));
+                    if (jlo) { y00 = 0.f; y01 = 0.f; }
+                    if (jhi) { y10 = 0.f; y11 = 0.f; }
+                    const float t1 = y00 * (1.f - a) + y10 * a;
+                    const float t2 = y01 * (1.f - a) + y11 * a;
+                    y[m] = t1 * (1.f - b[m]) + t2 * b[m];
+                }
+            }
+        }
+    } else {
+        // nearest: the index clamped to [-1, n] keeps the conversion defined
+        const int j = (int)floorf(fminf(fmaxf(fj + 0.5f, -1.f), (float)c.nu));
+        const bool uin = j >= 0 && j < (int)c.nu;
+        const int col = min(max(j, 0), (int)c.nu - 1) * (int32_t)c.s1;
+        Tp v[kZ];
+        bool inr[kZ];
+#pragma unroll
+        for (int m = 0; m < kZ; m++) {
+            cv[m] = __builtin_fmaf(z[m], dcv, cv0);
+            const int k = (int)floorf(fminf(fmaxf(cv[m] + M.hv + 0.5f, -1.f), (float)c.nv));
+            inr[m] = uin && k >= 0 && k < (int)c.nv;
+            const int kk = min(max(k, 0), (int)c.nv - 1);
+            v[m] = img[col + kk * (int32_t)c.s2];
+        }
+#pragma unroll
+        for (int m = 0; m < kZ; m++) y[m] = inr[m] ? to_f32(v[m]) : 0.f;
+    }
+#pragma unroll
+    for (int m = 0; m < kZ; m++) acc[m] += mult == 0.f ? 0.f : soften(c, M, cu, cv[m], y[m] * mult);
+}
+
 template <typename Tp>
 __device__ float bp_one_view(const BpCfg& c, const ViewMap& m, const Tp* __restrict__ img,
                              const float x[3]) {
@@ -477,80 +591,122 @@ __device__ float bp_one_view(const BpCfg& c, const ViewMap& m, const Tp* __restr
     return soften(c, m, cu, cv, val);
 }
 
-template <int kBy, typename Tv, typename Tp, typename Tm>
+// One thread per kZ voxels of a column, kBz apart along z (a wave holds kBz
+// consecutive voxels at each of its kZ heights).  kZ > 1 only for the
+// interpolating branch without sub-voxel sampling: a batch of views that are
+// all z-coherent takes bp_zcol_view, any other batch the per-voxel code.
+template <int kBy, int kZ, typename Tv, typename Tp, typename Tm>
 __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
     BpCfg c, const ap_proj_spec* __restrict__ specs, uint32_t num_projs,
     Tv* __restrict__ vol, int64_t vs0, int64_t vs1, int64_t vs2,
     const Tm* __restrict__ mult, int64_t ms0, int64_t ms1, int64_t ms2,
     const Tp* __restrict__ sino) {
     __shared__ ViewMap maps[kViewBatch];
-    const uint32_t ck = blockIdx.x * kBz + threadIdx.x;
+    const uint32_t ck0 = blockIdx.x * (kBz * kZ) + threadIdx.x;
     const uint32_t cj = blockIdx.y * kBy + threadIdx.y;
     const uint32_t ci = blockIdx.z;
-    const bool in = ck < c.shp[2] && cj < c.shp[1] && ci < c.shp[0];
-    const uint32_t gi = ci + c.off[0], gj = cj + c.off[1], gk = ck + c.off[2];
-    const float xc[3] = {c.c000[0] + (float)gi * c.w, c.c000[1] + (float)gj * c.w,
-                         c.c000[2] + (float)gk * c.w};
-    bool active = in;
-    if (c.mask && xc[0] * xc[0] + xc[1] * xc[1] > c.mask_r2) active = false;
+    const uint32_t gi = ci + c.off[0], gj = cj + c.off[1];
+    const float x0 = c.c000[0] + (float)gi * c.w, x1 = c.c000[1] + (float)gj * c.w;
+    bool col_in = cj < c.shp[1] && ci < c.shp[0];
+    if (c.mask && x0 * x0 + x1 * x1 > c.mask_r2) col_in = false;
+    float z[kZ], acc[kZ];
+    bool act[kZ];
+    bool any = false;
+#pragma unroll
+    for (int m = 0; m < kZ; m++) {
+        const uint32_t ck = ck0 + (uint32_t)(kBz * m);
+        z[m] = c.c000[2] + (float)(ck + c.off[2]) * c.w;
+        act[m] = col_in && ck < c.shp[2];
+        acc[m] = 0.f;
+        any = any || act[m];
+    }
     const uint32_t nss = c.ssx * c.ssy * c.ssz;
-    // sub-voxel accumulators live in registers for the common ss=1 case
-    float acc1 = 0.f;
-    float ax = 0.f;
+    const bool zcol = kZ > 1 && !c.exact && nss == 1 && (c.bilinear ? c.pair : c.off32);
     const int tid = threadIdx.y * kBz + threadIdx.x;
     for (uint32_t p0 = 0; p0 < num_projs; p0 += kViewBatch) {
         const uint32_t nb = min((uint32_t)kViewBatch, num_projs - p0);
         __syncthreads();
-        if (tid < (int)nb) make_view_map(specs, p0 + tid, maps[tid]);
-        __syncthreads();
-        if (!active) continue;
-        if (nss == 1) {
+        bool coherent = true;
+        if (tid < (int)nb) {
+            make_view_map(specs, p0 + tid, maps[tid]);
+            coherent = view_z_coherent(c, maps[tid]);
+        }
+        // every view of the batch z-coherent (the barrier publishes the maps)
+        const bool batch_z = __syncthreads_and(coherent) != 0;
+        if (!any || nss != 1) continue;
+        if (zcol && batch_z) {
+            for (uint32_t q = 0; q < nb; q++)
+                bp_zcol_view<kZ>(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, x0, x1, z, act, acc);
+            continue;
+        }
+        // per voxel: one copy of the view loop, the voxel's height and sum taken
+        // from (and put back into) its registers by selects
+#pragma unroll 1
+        for (int m = 0; m < kZ; m++) {
+            float zm = z[0], am = 0.f;
+            bool on = act[0];
+#pragma unroll
+            for (int i = 1; i < kZ; i++)
+                if (m == i) { zm = z[i]; on = act[i]; }
+            if (!on) continue;
+            const float xc[3] = {x0, x1, zm};
             if (c.exact) {
                 for (uint32_t q = 0; q < nb; q++)
-                    acc1 += bp_one_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
+                    am += bp_one_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
             } else if (c.grouped) {
                 uint32_t q = 0;
                 for (; q + 4 <= nb; q += 4)
-                    bp_fast_views4(c, maps + q, sino + (int64_t)(p0 + q) * c.s0, c.s0, xc, acc1);
+                    bp_fast_views4(c, maps + q, sino + (int64_t)(p0 + q) * c.s0, c.s0, xc, am);
                 for (; q < nb; q++)
-                    acc1 += bp_fast_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
+                    am += bp_fast_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
             } else {
                 for (uint32_t q = 0; q < nb; q++)
-                    acc1 += bp_fast_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
+                    am += bp_fast_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
             }
+#pragma unroll
+            for (int i = 0; i < kZ; i++)
+                if (m == i) acc[i] += am;
         }
     }
-    if (nss != 1 && active) {
-        // general sub-voxel sampling: views re-derived per sub-voxel
+    if (nss != 1) {
+        // general sub-voxel sampling (kZ = 1 only): views re-derived per sub-voxel
         // (CB_backproject.cu:377-383 loop order: x, y, z outermost to innermost)
-        ax = 0.f;
-        for (uint32_t si = 0; si < c.ssx; si++) {
-            float ay = 0.f;
-            for (uint32_t sj = 0; sj < c.ssy; sj++) {
-                float az = 0.f;
-                for (uint32_t sk = 0; sk < c.ssz; sk++) {
-                    const float xs[3] = {xc[0] + (-0.5f + (0.5f + si) / c.ssx) * c.w,
-                                         xc[1] + (-0.5f + (0.5f + sj) / c.ssy) * c.w,
-                                         xc[2] + (-0.5f + (0.5f + sk) / c.ssz) * c.w};
-                    float acc = 0.f;
-                    for (uint32_t p = 0; p < num_projs; p++) {
-                        ViewMap m;
-                        make_view_map(specs, p, m);
-                        acc += bp_one_view(c, m, sino + (int64_t)p * c.s0, xs);
+#pragma unroll
+        for (int m = 0; m < kZ; m++) {
+            if (!act[m]) continue;
+            float ax = 0.f;
+            for (uint32_t si = 0; si < c.ssx; si++) {
+                float ay = 0.f;
+                for (uint32_t sj = 0; sj < c.ssy; sj++) {
+                    float az = 0.f;
+                    for (uint32_t sk = 0; sk < c.ssz; sk++) {
+                        const float xs[3] = {x0 + (-0.5f + (0.5f + si) / c.ssx) * c.w,
+                                             x1 + (-0.5f + (0.5f + sj) / c.ssy) * c.w,
+                                             z[m] + (-0.5f + (0.5f + sk) / c.ssz) * c.w};
+                        float a = 0.f;
+                        for (uint32_t p = 0; p < num_projs; p++) {
+                            ViewMap vm;
+                            make_view_map(specs, p, vm);
+                            a += bp_one_view(c, vm, sino + (int64_t)p * c.s0, xs);
+                        }
+                        az += a;
                     }
-                    az += acc;
+                    ay += az / c.ssz;
                 }
-                ay += az / c.ssz;
+                ax += ay / c.ssy;
             }
-            ax += ay / c.ssy;
+            acc[m] = ax / c.ssx;
         }
-        acc1 = ax / c.ssx;
     }
-    if (!active || acc1 == 0.f) return;
-    float mval = 1.f;
-    if (c.use_mult) mval = to_f32(mult[(int64_t)gi * ms0 + (int64_t)gj * ms1 + (int64_t)gk * ms2]);
-    const int64_t o = (int64_t)gi * vs0 + (int64_t)gj * vs1 + (int64_t)gk * vs2;
-    vol[o] = from_f32<Tv>(to_f32(vol[o]) + acc1 * mval);
+#pragma unroll
+    for (int m = 0; m < kZ; m++) {
+        if (!act[m] || acc[m] == 0.f) continue;
+        const int64_t gk = (int64_t)(ck0 + (uint32_t)(kBz * m) + c.off[2]);
+        float mval = 1.f;
+        if (c.use_mult) mval = to_f32(mult[(int64_t)gi * ms0 + (int64_t)gj * ms1 + gk * ms2]);
+        const int64_t o = (int64_t)gi * vs0 + (int64_t)gj * vs1 + gk * vs2;
+        vol[o] = from_f32<Tv>(to_f32(vol[o]) + acc[m] * mval);
+    }
 }
 
 // the window s at sn = sin|t|
@@ -767,8 +923,13 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
                         (double)(c.nv ? c.nv - 1 : 0) * (double)c.s2 < 2147483647.0;
     const int var = ap_tune_variant(AP_TUNE_BACKPROJECT);
     const uint32_t rows_y = (uint32_t)kBpRowsY[var];
+    // four voxels per thread along z where a column is taller than two waves'
+    // worth and the branch can share a view's u part between them
+    const uint32_t nss = c.ssx * c.ssy * c.ssz;
+    const bool z4 = !c.exact && nss == 1 && c.shp[2] > 2 * (uint32_t)kBz;
+    const uint32_t zper = z4 ? 4u * kBz : (uint32_t)kBz;
     dim3 block(kBz, rows_y, 1);
-    dim3 grid((c.shp[2] + kBz - 1) / kBz, (c.shp[1] + rows_y - 1) / rows_y, c.shp[0]);
+    dim3 grid((c.shp[2] + zper - 1) / zper, (c.shp[1] + rows_y - 1) / rows_y, c.shp[0]);
     const int32_t mdt = multiplier ? mult_desc->dtype : vol_desc->dtype;
     const int64_t* ms = multiplier ? mult_desc->strides : vol_desc->strides;
     return dispatch_dtype(vol_desc->dtype, [&](auto tv) -> int32_t {
@@ -796,9 +957,15 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
                                            vol_desc->strides[0], vol_desc->strides[1], vol_desc->strides[2],
                                            (const Tm*)multiplier, ms[0], ms[1], ms[2], (const Tp*)sino);
                     };
-                    if (var == 0) launch(k_backproject_voxel<kBpRowsY[0], Tv, Tp, Tm>);
-                    if (var == 1) launch(k_backproject_voxel<kBpRowsY[1], Tv, Tp, Tm>);
-                    if (var == 2) launch(k_backproject_voxel<kBpRowsY[2], Tv, Tp, Tm>);
+                    if (z4) {
+                        if (var == 0) launch(k_backproject_voxel<kBpRowsY[0], 4, Tv, Tp, Tm>);
+                        if (var == 1) launch(k_backproject_voxel<kBpRowsY[1], 4, Tv, Tp, Tm>);
+                        if (var == 2) launch(k_backproject_voxel<kBpRowsY[2], 4, Tv, Tp, Tm>);
+                    } else {
+                        if (var == 0) launch(k_backproject_voxel<kBpRowsY[0], 1, Tv, Tp, Tm>);
+                        if (var == 1) launch(k_backproject_voxel<kBpRowsY[1], 1, Tv, Tp, Tm>);
+                        if (var == 2) launch(k_backproject_voxel<kBpRowsY[2], 1, Tv, Tp, Tm>);
+                    }
                     return ap_check_launch("backproject_voxel");
                 }
             });

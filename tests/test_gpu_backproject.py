@@ -293,3 +293,82 @@ def test_backprojector_multiplier_and_16bit_volumes(device):
         print(f"multiplier case {dt}: rel L2 {err:.3e}")
         assert err <= tol
     del torch
+
+
+TALL_CASES = {
+    "fast_bilinear": BP_CASES["fast_bilinear"],
+    "fast_nearest_mask_soft": dict(fast_approx=True, cyl_mask_rad=1.0, soften_det_edges_proportion=0.05),
+    "fdk_direct_zval": BP_CASES["fdk_direct_zval"],
+    "preset2_zpow": dict(fast_approx=True, approx_mode_direct_raycast=True, approx_mode_bilinear_interpolate=True,
+                         Z_on_val_factor_power=-1, Z_on_val_factor_val=40.0, backproj_weighting_preset=2),
+    "zpow_L": dict(fast_approx=True, approx_mode_bilinear_interpolate=True, Z_on_L_factor_power=2),
+}
+
+
+def _tall_geometry(kind: str, views: int, n: int) -> cbfp.ProjGeom:
+    circ = ph.circular_geometry(views, n, n, 1.0, R=40.0, L=80.0)
+    tilt = ph.tilted_geometry(views, n, 1.0, inplane_deg=4.0, outplane_deg=-3.0, offset_uv=(1.5, -2.0),
+                              R=40.0, L=80.0)
+    if kind == "circular":
+        return circ
+    if kind == "tilted":
+        return tilt
+    # the first 64 views (one batch) upright, the rest tilted
+    u = np.broadcast_to(circ.det_uvec, tilt.det_uvec.shape).copy()
+    v = np.broadcast_to(circ.det_vvec, tilt.det_vvec.shape).copy()
+    d = np.broadcast_to(circ.det_pos, tilt.det_pos.shape).copy()
+    u[64:], v[64:], d[64:] = tilt.det_uvec[64:], tilt.det_vvec[64:], tilt.det_pos[64:]
+    return cbfp.ProjGeom(source_pos=tilt.source_pos, det_pos=d, det_uvec=u, det_vvec=v, nu=n, nv=n)
+
+
+@pytest.mark.parametrize("geom", ["circular", "tilted", "mixed"])
+@pytest.mark.parametrize("name", list(TALL_CASES))
+def test_fast_backprojector_tall_columns(device, name, geom):
+    # Volumes more than two waves tall take four voxels per thread along z.  A
+    # batch of 64 views whose panels are upright (z-coherent maps) shares each
+    # view's u part between a thread's voxels; any other batch evaluates each
+    # voxel on its own ("mixed": one batch of each).  150 planes leave the
+    # fourth voxel of every thread and part of the third outside the volume,
+    # and the volume is taller than the field of view, so taps leave the
+    # detector's first and last rows.
+    rng = np.random.default_rng(300 + len(name) + 7 * len(geom))
+    views, n = 70, 40
+    shape, w = (10, 9, 150), 0.25
+    pg = _tall_geometry(geom, views, n)
+    y = rng.uniform(0, 1, (views, n, n)).astype(np.float32)
+    cfg = cbfp.ConfigBackproject(**TALL_CASES[name])
+    v = Volume(np.zeros(shape, np.float32), voxwid=w)
+    cbfp.backproject(cfg, v, cbfp.ProjectionSet(pg, data=to_gpu(y)), block=True)
+    want = orc.backproject(cfg.contents, pg.to_proj_spec_array(), chunk_for(shape, w), np.zeros(shape), y)
+    err = rel_l2(v.as_numpy(), want)
+    print(f"tall-column backproject[{name}, {geom}] rel L2: {err:.3e}")
+    assert (want == 0).any() and np.abs(want).max() > 0
+    assert err <= 2e-5
+
+
+@pytest.mark.parametrize("layout", ["u_contiguous", "fp16"])
+def test_fast_backprojector_tall_columns_layouts(device, layout):
+    # the same on upright panels with detector data that is not v-contiguous
+    # (no paired taps: each voxel on its own) and with fp16 pairs
+    import torch
+
+    rng = np.random.default_rng(77)
+    views, n = 70, 40
+    shape, w = (10, 9, 200), 0.25
+    pg = _tall_geometry("circular", views, n)
+    y = rng.uniform(-1, 1, (views, n, n)).astype(np.float32)
+    if layout == "u_contiguous":
+        data = to_gpu(y).transpose(1, 2).contiguous().transpose(1, 2)
+        assert data.stride(2) != 1
+        yy = y
+    else:
+        data = to_gpu(y).to(torch.float16)
+        yy = data.float().cpu().numpy()
+    cfg = cbfp.ConfigBackproject(fast_approx=True, approx_mode_bilinear_interpolate=True)
+    v = Volume(np.zeros(shape, np.float32), voxwid=w)
+    cbfp.backproject(cfg, v, cbfp.ProjectionSet(pg, data=data), block=True)
+    want = orc.backproject(cfg.contents, pg.to_proj_spec_array(), chunk_for(shape, w), np.zeros(shape), yy)
+    err = rel_l2(v.as_numpy(), want)
+    print(f"tall-column backproject ({layout}) rel L2 {err:.3e}")
+    assert np.abs(want).max() > 0
+    assert err <= 2e-5
