@@ -122,7 +122,7 @@ struct BpCfg {
     float w3;             // voxwid^3
     bool off32;           // detector offsets of one view fit 32 bits
     bool pair;            // v contiguous (s2 == 1), nv >= 2, 32-bit offsets: rows k, k+1 in one load
-    bool grouped;         // bp_fast_views4 applies: offsets of 4 views fit 32 bits (and pairs if bilinear)
+    bool grouped;         // bp_col_views over 4 views applies: offsets of 4 views fit 32 bits (and pairs if bilinear)
     int zl, zv;
     uint32_t ssx, ssy, ssz;
     float w;              // voxwid
@@ -320,112 +320,6 @@ __device__ __forceinline__ float sample_det(const BpCfg& c, const ViewMap& m, co
     return y;
 }
 
-// Four consecutive views of the interpolating branch with their detector
-// reads issued together, so that a lane has four (bilinear: eight) gathers in
-// flight instead of one.  The arithmetic of each view is that of
-// bp_fast_view / sample_det, and the views are added to `acc` in order, so
-// the sums are bit-identical.  Reads use clamped in-range addresses and are
-// zeroed afterwards where the original reads nothing.  Needs 32-bit detector
-// offsets, and for bilinear taps the paired layout (c.pair).
-template <typename Tp>
-__device__ __forceinline__ void bp_fast_views4(const BpCfg& c, const ViewMap* m, const Tp* __restrict__ img,
-                                               int64_t s0, const float x[3], float& acc) {
-    float cu[4], cv[4], mult[4], ta[4], tb[4];
-    int o0[4], o1[4];
-    bool inr[4], klo[4], khi[4], jlo[4], jhi[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const ViewMap& M = m[i];
-        const float wx = M.W[0] * x[0] + M.W[1] * x[1] + M.W[2] * x[2] + M.w0;
-        const float iw = __builtin_amdgcn_rcpf(wx);
-        cu[i] = (M.Au[0] * x[0] + M.Au[1] * x[1] + M.Au[2] * x[2] + M.au) * iw;
-        cv[i] = (M.Av[0] * x[0] + M.Av[1] * x[1] + M.Av[2] * x[2] + M.av) * iw;
-        float mu = c.direct ? 1.f : M.kq * c.w3 * (iw * iw);
-        const bool pre = c.direct && c.preset != 0;
-        if (c.zl != 0 || c.zv != 0 || pre) {
-            const float zz = M.Zc[0] * x[0] + M.Zc[1] * x[1] + M.Zc[2] * x[2] + M.z0;
-            if (c.zl != 0) mu *= powi(zz * M.inv_Lnorm, c.zl);
-            if (c.zv != 0) mu *= powi(zz * c.inv_zval, c.zv);
-            if (pre) mu = apply_preset(c, M, x, zz, mu);
-        }
-        mult[i] = mu;
-        const float fj = cu[i] + M.hu, fk = cv[i] + M.hv;
-        const int64_t vo = (int64_t)i * s0;
-        if (c.bilinear) {
-            inr[i] = fj > -1.f && fk > -1.f && fj < (float)c.nu && fk < (float)c.nv;
-            const float fjc = inr[i] ? fj : 0.f, fkc = inr[i] ? fk : 0.f;   // finite before the conversion
-            const float fl_j = floorf(fjc), fl_k = floorf(fkc);
-            const int j0 = (int)fl_j, k0 = (int)fl_k;
-            ta[i] = fjc - fl_j;
-            tb[i] = fkc - fl_k;
-            const int ja = max(j0, 0), jb = min(j0 + 1, (int)c.nu - 1);
-            const int kc = min(max(k0, 0), (int)c.nv - 2);
-            o0[i] = (int)(vo + ja * (int32_t)c.s1 + kc);
-            o1[i] = (int)(vo + jb * (int32_t)c.s1 + kc);
-            klo[i] = k0 < 0;
-            khi[i] = k0 + 1 >= (int)c.nv;
-            jlo[i] = j0 < 0;
-            jhi[i] = j0 + 1 >= (int)c.nu;
-        } else {
-            // nearest: the index clamped to [-1, n] keeps the conversion defined
-            const int j = (int)floorf(fminf(fmaxf(fj + 0.5f, -1.f), (float)c.nu));
-            const int k = (int)floorf(fminf(fmaxf(fk + 0.5f, -1.f), (float)c.nv));
-            inr[i] = j >= 0 && k >= 0 && j < (int)c.nu && k < (int)c.nv;
-            const int jc = min(max(j, 0), (int)c.nu - 1), kk = min(max(k, 0), (int)c.nv - 1);
-            o0[i] = (int)(vo + jc * (int32_t)c.s1 + kk * (int32_t)c.s2);
-        }
-    }
-    if (c.bilinear) {
-        TapPair<Tp> pa[4], pb[4];
-        bool edge = false;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            pa[i] = *reinterpret_cast<const TapPair<Tp>*>(img + o0[i]);
-            pb[i] = *reinterpret_cast<const TapPair<Tp>*>(img + o1[i]);
-            edge = edge || !inr[i] || klo[i] || khi[i] || jlo[i] || jhi[i];
-        }
-        if (__ballot(edge) == 0) {
-            // no tap of the wave's 4 views falls off the detector: the blend of
-            // the two loaded pairs, without the edge selects (the same values)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float a = ta[i], b = tb[i];
-                const float t1 = to_f32(pa[i].lo) * (1.f - a) + to_f32(pb[i].lo) * a;
-                const float t2 = to_f32(pa[i].hi) * (1.f - a) + to_f32(pb[i].hi) * a;
-                const float y = t1 * (1.f - b) + t2 * b;
-                acc += mult[i] == 0.f ? 0.f : soften(c, m[i], cu[i], cv[i], y * mult[i]);
-            }
-            return;
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            float y = 0.f;
-            if (inr[i]) {
-                float y00 = klo[i] ? 0.f : (khi[i] ? to_f32(pa[i].hi) : to_f32(pa[i].lo));
-                float y01 = khi[i] ? 0.f : (klo[i] ? to_f32(pa[i].lo) : to_f32(pa[i].hi));
-                float y10 = klo[i] ? 0.f : (khi[i] ? to_f32(pb[i].hi) : to_f32(pb[i].lo));
-                float y11 = khi[i] ? 0.f : (klo[i] ? to_f32(pb[i].lo) : to_f32(pb[i].hi));
-                if (jlo[i]) { y00 = 0.f; y01 = 0.f; }
-                if (jhi[i]) { y10 = 0.f; y11 = 0.f; }
-                const float a = ta[i], b = tb[i];
-                const float t1 = y00 * (1.f - a) + y10 * a;
-                const float t2 = y01 * (1.f - a) + y11 * a;
-                y = t1 * (1.f - b) + t2 * b;
-            }
-            acc += mult[i] == 0.f ? 0.f : soften(c, m[i], cu[i], cv[i], y * mult[i]);
-        }
-    } else {
-        float yv[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) yv[i] = to_f32(img[o0[i]]);
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float y = inr[i] ? yv[i] : 0.f;
-            acc += mult[i] == 0.f ? 0.f : soften(c, m[i], cu[i], cv[i], y * mult[i]);
-        }
-    }
-}
-
 // A view is z-coherent when its projective map does not depend on z except
 // through cv: W.z = 0 and Au.z = 0 (detector normal and u axis horizontal),
 // and, where the Z weights or a weighting preset apply, Zc.z = 0 (horizontal
@@ -436,108 +330,183 @@ __device__ __forceinline__ bool view_z_coherent(const BpCfg& c, const ViewMap& m
     return m.W[2] == 0.f && m.Au[2] == 0.f && (!zdep || m.Zc[2] == 0.f);
 }
 
-// The interpolating branch for one z-coherent view at kZ voxels of one
-// column (x, y) at heights z[m]: the u part -- denominator, cu, the weight,
-// the u taps -- once, then per voxel cv = cv(0) + z dcv, its v taps and the
-// blend of bp_fast_view / sample_det.  Reads use clamped in-range addresses
-// and are zeroed where sample_det reads nothing.  Needs 32-bit detector
-// offsets, and for bilinear taps the paired layout (c.pair).
-template <int kZ, typename Tp>
-__device__ __forceinline__ void bp_zcol_view(const BpCfg& c, const ViewMap& M, const Tp* __restrict__ img,
-                                             float x0, float x1, const float (&z)[kZ], const bool (&act)[kZ],
-                                             float (&acc)[kZ]) {
-    const float wx = M.W[0] * x0 + M.W[1] * x1 + M.w0;
-    const float iw = __builtin_amdgcn_rcpf(wx);
-    const float cu = (M.Au[0] * x0 + M.Au[1] * x1 + M.au) * iw;
-    const float cv0 = (M.Av[0] * x0 + M.Av[1] * x1 + M.av) * iw;
-    const float dcv = M.Av[2] * iw;
-    float mult = c.direct ? 1.f : M.kq * c.w3 * (iw * iw);
-    const bool pre = c.direct && c.preset != 0;
-    if (c.zl != 0 || c.zv != 0 || pre) {
-        const float zz = M.Zc[0] * x0 + M.Zc[1] * x1 + M.z0;
-        if (c.zl != 0) mult *= powi(zz * M.inv_Lnorm, c.zl);
-        if (c.zv != 0) mult *= powi(zz * c.inv_zval, c.zv);
-        if (pre) {
-            const float xp[3] = {x0, x1, 0.f};   // the presets use x and y only
-            mult = apply_preset(c, M, xp, zz, mult);
-        }
-    }
-    const float fj = cu + M.hu;
-    float cv[kZ], y[kZ];
-    if (c.bilinear) {
-        const bool uin = fj > -1.f && fj < (float)c.nu;
-        const float fjc = uin ? fj : 0.f;   // finite before the conversion
-        const float fl_j = floorf(fjc);
-        const int j0 = (int)fl_j;
-        const float a = fjc - fl_j;
-        const int ja = max(j0, 0), jb = min(j0 + 1, (int)c.nu - 1);
-        const int ca = ja * (int32_t)c.s1, cb = jb * (int32_t)c.s1;
-        const bool jlo = j0 < 0, jhi = j0 + 1 >= (int)c.nu;
-        TapPair<Tp> pa[kZ], pb[kZ];
-        float b[kZ];
-        bool inr[kZ], klo[kZ], khi[kZ];
-        bool edge = false;
+// The interpolating branch for kV consecutive views at kZ voxels of one
+// column (x, y) at heights z[m], all detector reads issued before the
+// blends (kV kZ, bilinear 2 kV kZ, gathers in flight per lane).  With kCoh
+// (every view z-coherent) a view's u part -- denominator, cu, the weight, the
+// u taps -- is formed once and cv = cv(0) + z dcv per voxel; otherwise each
+// voxel forms the whole map as bp_fast_view does.  The blends and weights are
+// those of bp_fast_view / sample_det, and each voxel adds its views in order,
+// so a voxel's sum does not depend on kV or kZ.  Reads use clamped in-range
+// addresses and are zeroed where sample_det reads nothing.  Needs 32-bit
+// offsets over the kV views (view v at img + v s0), and for bilinear taps the
+// paired layout (c.pair).
+template <int kV, int kZ, bool kCoh, typename Tp>
+__device__ __forceinline__ void bp_col_views(const BpCfg& c, const ViewMap* Ms, const Tp* __restrict__ img,
+                                             int64_t s0, float x0, float x1, const float (&z)[kZ],
+                                             uint32_t act, float (&acc)[kZ]) {
+    float cu[kV][kZ], cv[kV][kZ], mult[kV][kZ], y[kV][kZ];
 #pragma unroll
-        for (int m = 0; m < kZ; m++) {
-            cv[m] = __builtin_fmaf(z[m], dcv, cv0);
-            const float fk = cv[m] + M.hv;
-            inr[m] = uin && fk > -1.f && fk < (float)c.nv;
-            const float fkc = inr[m] ? fk : 0.f;
-            const float fl_k = floorf(fkc);
-            const int k0 = (int)fl_k;
-            b[m] = fkc - fl_k;
-            const int kc = min(max(k0, 0), (int)c.nv - 2);
-            pa[m] = *reinterpret_cast<const TapPair<Tp>*>(img + (ca + kc));
-            pb[m] = *reinterpret_cast<const TapPair<Tp>*>(img + (cb + kc));
-            klo[m] = k0 < 0;
-            khi[m] = k0 + 1 >= (int)c.nv;
-            edge = edge || (act[m] && (!inr[m] || klo[m] || khi[m] || jlo || jhi));
-        }
-        if (__ballot(edge) == 0) {
-            // no tap of the wave's active voxels falls off the detector
+    for (int v = 0; v < kV; v++) {
+        const ViewMap& M = Ms[v];
+        const bool pre = c.direct && c.preset != 0;
+        if constexpr (kCoh) {
+            const float wx = M.W[0] * x0 + M.W[1] * x1 + M.w0;
+            const float iw = __builtin_amdgcn_rcpf(wx);
+            const float cuv = (M.Au[0] * x0 + M.Au[1] * x1 + M.au) * iw;
+            const float cv0 = (M.Av[0] * x0 + M.Av[1] * x1 + M.av) * iw;
+            const float dcv = M.Av[2] * iw;
+            float mu = c.direct ? 1.f : M.kq * c.w3 * (iw * iw);
+            if (c.zl != 0 || c.zv != 0 || pre) {
+                const float zz = M.Zc[0] * x0 + M.Zc[1] * x1 + M.z0;
+                if (c.zl != 0) mu *= powi(zz * M.inv_Lnorm, c.zl);
+                if (c.zv != 0) mu *= powi(zz * c.inv_zval, c.zv);
+                if (pre) {
+                    const float xp[3] = {x0, x1, 0.f};   // the presets use x and y only
+                    mu = apply_preset(c, M, xp, zz, mu);
+                }
+            }
 #pragma unroll
             for (int m = 0; m < kZ; m++) {
-                const float t1 = to_f32(pa[m].lo) * (1.f - a) + to_f32(pb[m].lo) * a;
-                const float t2 = to_f32(pa[m].hi) * (1.f - a) + to_f32(pb[m].hi) * a;
-                y[m] = t1 * (1.f - b[m]) + t2 * b[m];
+                cu[v][m] = cuv;
+                cv[v][m] = __builtin_fmaf(z[m], dcv, cv0);
+                mult[v][m] = mu;
             }
         } else {
 #pragma unroll
             for (int m = 0; m < kZ; m++) {
-                y[m] = 0.f;
-                if (inr[m]) {
-                    float y00 = klo[m] ? 0.f : (khi[m] ? to_f32(pa[m].hi) : to_f32(pa[m].lo));
-                    float y01 = khi[m] ? 0.f : (klo[m] ? to_f32(pa[m].lo) : to_f32(pa[m].hi));
-                    float y10 = klo[m] ? 0.f : (khi[m] ? to_f32(pb[m].hi) : to_f32(pb[m].lo));
-                    float y11 = khi[m] ? 0.f : (klo[m] ? to_f32(pb[m].lo) : to_f32(pb[m].hi));
-                    if (jlo) { y00 = 0.f; y01 = 0.f; }
-                    if (jhi) { y10 = 0.f; y11 = 0.f; }
-                    const float t1 = y00 * (1.f - a) + y10 * a;
-                    const float t2 = y01 * (1.f - a) + y11 * a;
-                    y[m] = t1 * (1.f - b[m]) + t2 * b[m];
+                const float x[3] = {x0, x1, z[m]};
+                const float wx = M.W[0] * x[0] + M.W[1] * x[1] + M.W[2] * x[2] + M.w0;
+                const float iw = __builtin_amdgcn_rcpf(wx);
+                cu[v][m] = (M.Au[0] * x[0] + M.Au[1] * x[1] + M.Au[2] * x[2] + M.au) * iw;
+                cv[v][m] = (M.Av[0] * x[0] + M.Av[1] * x[1] + M.Av[2] * x[2] + M.av) * iw;
+                float mu = c.direct ? 1.f : M.kq * c.w3 * (iw * iw);
+                if (c.zl != 0 || c.zv != 0 || pre) {
+                    const float zz = M.Zc[0] * x[0] + M.Zc[1] * x[1] + M.Zc[2] * x[2] + M.z0;
+                    if (c.zl != 0) mu *= powi(zz * M.inv_Lnorm, c.zl);
+                    if (c.zv != 0) mu *= powi(zz * c.inv_zval, c.zv);
+                    if (pre) mu = apply_preset(c, M, x, zz, mu);
                 }
+                mult[v][m] = mu;
             }
         }
-    } else {
-        // nearest: the index clamped to [-1, n] keeps the conversion defined
-        const int j = (int)floorf(fminf(fmaxf(fj + 0.5f, -1.f), (float)c.nu));
-        const bool uin = j >= 0 && j < (int)c.nu;
-        const int col = min(max(j, 0), (int)c.nu - 1) * (int32_t)c.s1;
-        Tp v[kZ];
-        bool inr[kZ];
+    }
+    if (c.bilinear) {
+        TapPair<Tp> pa[kV][kZ], pb[kV][kZ];
+        float ta[kV][kZ], tb[kV][kZ];
+        // tap indices and the in-range flag for the blend with edges.  Per-voxel
+        // maps keep them as integers (opaque copies): as flags across the branch
+        // they would be lane masks, held in scalar registers; a z-coherent view
+        // shares its u taps and needs fewer
+        int j0v[kV][kZ], k0v[kV][kZ], inv[kV][kZ];
+        bool edge = false;
 #pragma unroll
-        for (int m = 0; m < kZ; m++) {
-            cv[m] = __builtin_fmaf(z[m], dcv, cv0);
-            const int k = (int)floorf(fminf(fmaxf(cv[m] + M.hv + 0.5f, -1.f), (float)c.nv));
-            inr[m] = uin && k >= 0 && k < (int)c.nv;
-            const int kk = min(max(k, 0), (int)c.nv - 1);
-            v[m] = img[col + kk * (int32_t)c.s2];
+        for (int v = 0; v < kV; v++) {
+            const ViewMap& M = Ms[v];
+            const int vo = v * (int32_t)s0;
+            // a z-coherent view's u taps, once: where the voxel's sample is in
+            // range they equal the per-voxel ones, and elsewhere it reads 0
+            const float fju = cu[v][0] + M.hu;
+            const bool uin = fju > -1.f && fju < (float)c.nu;
+            const float fjcu = uin ? fju : 0.f;
+            const float flu = floorf(fjcu);
+            const int j0u = (int)flu;
+            const float tau = fjcu - flu;
+#pragma unroll
+            for (int m = 0; m < kZ; m++) {
+                const float fk = cv[v][m] + M.hv;
+                bool inr;
+                int j0;
+                if constexpr (kCoh) {
+                    inr = uin && fk > -1.f && fk < (float)c.nv;
+                    j0 = j0u;
+                    ta[v][m] = tau;
+                } else {
+                    const float fj = cu[v][m] + M.hu;
+                    inr = fj > -1.f && fk > -1.f && fj < (float)c.nu && fk < (float)c.nv;
+                    const float fjc = inr ? fj : 0.f;   // finite before the conversion
+                    const float fl_j = floorf(fjc);
+                    j0 = (int)fl_j;
+                    ta[v][m] = fjc - fl_j;
+                }
+                const float fkc = inr ? fk : 0.f;
+                const float fl_k = floorf(fkc);
+                const int k0 = (int)fl_k;
+                tb[v][m] = fkc - fl_k;
+                const int ja = max(j0, 0), jb = min(j0 + 1, (int)c.nu - 1);
+                const int kc = min(max(k0, 0), (int)c.nv - 2);
+                pa[v][m] = *reinterpret_cast<const TapPair<Tp>*>(img + (vo + ja * (int32_t)c.s1 + kc));
+                pb[v][m] = *reinterpret_cast<const TapPair<Tp>*>(img + (vo + jb * (int32_t)c.s1 + kc));
+                edge = edge || (((act >> m) & 1u) &&
+                                (!inr || k0 < 0 || k0 + 1 >= (int)c.nv || j0 < 0 || j0 + 1 >= (int)c.nu));
+                j0v[v][m] = j0;
+                k0v[v][m] = k0;
+                inv[v][m] = inr ? 1 : 0;
+                if constexpr (!kCoh) asm volatile("" : "+v"(j0v[v][m]), "+v"(k0v[v][m]), "+v"(inv[v][m]));
+            }
+        }
+        if (__ballot(edge) == 0) {
+            // no tap of the wave's active voxels falls off the detector: the blend of
+            // the two loaded pairs without the edge selects (the same values)
+#pragma unroll
+            for (int v = 0; v < kV; v++)
+#pragma unroll
+                for (int m = 0; m < kZ; m++) {
+                    const float a = ta[v][m], b = tb[v][m];
+                    const float t1 = to_f32(pa[v][m].lo) * (1.f - a) + to_f32(pb[v][m].lo) * a;
+                    const float t2 = to_f32(pa[v][m].hi) * (1.f - a) + to_f32(pb[v][m].hi) * a;
+                    y[v][m] = t1 * (1.f - b) + t2 * b;
+                }
+        } else {
+#pragma unroll
+            for (int v = 0; v < kV; v++)
+#pragma unroll
+                for (int m = 0; m < kZ; m++) {
+                    y[v][m] = 0.f;
+                    if (inv[v][m] != 0) {
+                        const int j0 = j0v[v][m], k0 = k0v[v][m];
+                        const bool lo = k0 < 0, hi = k0 + 1 >= (int)c.nv;
+                        float y00 = lo ? 0.f : (hi ? to_f32(pa[v][m].hi) : to_f32(pa[v][m].lo));
+                        float y01 = hi ? 0.f : (lo ? to_f32(pa[v][m].lo) : to_f32(pa[v][m].hi));
+                        float y10 = lo ? 0.f : (hi ? to_f32(pb[v][m].hi) : to_f32(pb[v][m].lo));
+                        float y11 = hi ? 0.f : (lo ? to_f32(pb[v][m].lo) : to_f32(pb[v][m].hi));
+                        if (j0 < 0) { y00 = 0.f; y01 = 0.f; }
+                        if (j0 + 1 >= (int)c.nu) { y10 = 0.f; y11 = 0.f; }
+                        const float a = ta[v][m], b = tb[v][m];
+                        const float t1 = y00 * (1.f - a) + y10 * a;
+                        const float t2 = y01 * (1.f - a) + y11 * a;
+                        y[v][m] = t1 * (1.f - b) + t2 * b;
+                    }
+                }
+        }
+    } else {
+        Tp yv[kV][kZ];
+        bool inr[kV][kZ];
+#pragma unroll
+        for (int v = 0; v < kV; v++) {
+            const ViewMap& M = Ms[v];
+            const int vo = v * (int32_t)s0;
+#pragma unroll
+            for (int m = 0; m < kZ; m++) {
+                // nearest: the index clamped to [-1, n] keeps the conversion defined
+                const float fj = cu[v][m] + M.hu, fk = cv[v][m] + M.hv;
+                const int j = (int)floorf(fminf(fmaxf(fj + 0.5f, -1.f), (float)c.nu));
+                const int k = (int)floorf(fminf(fmaxf(fk + 0.5f, -1.f), (float)c.nv));
+                inr[v][m] = j >= 0 && k >= 0 && j < (int)c.nu && k < (int)c.nv;
+                const int jc = min(max(j, 0), (int)c.nu - 1), kk = min(max(k, 0), (int)c.nv - 1);
+                yv[v][m] = img[vo + jc * (int32_t)c.s1 + kk * (int32_t)c.s2];
+            }
         }
 #pragma unroll
-        for (int m = 0; m < kZ; m++) y[m] = inr[m] ? to_f32(v[m]) : 0.f;
+        for (int v = 0; v < kV; v++)
+#pragma unroll
+            for (int m = 0; m < kZ; m++) y[v][m] = inr[v][m] ? to_f32(yv[v][m]) : 0.f;
     }
 #pragma unroll
-    for (int m = 0; m < kZ; m++) acc[m] += mult == 0.f ? 0.f : soften(c, M, cu, cv[m], y[m] * mult);
+    for (int v = 0; v < kV; v++)
+#pragma unroll
+        for (int m = 0; m < kZ; m++)
+            acc[m] += mult[v][m] == 0.f ? 0.f : soften(c, Ms[v], cu[v][m], cv[v][m], y[v][m] * mult[v][m]);
 }
 
 template <typename Tp>
@@ -592,9 +561,10 @@ __device__ float bp_one_view(const BpCfg& c, const ViewMap& m, const Tp* __restr
 }
 
 // One thread per kZ voxels of a column, kBz apart along z (a wave holds kBz
-// consecutive voxels at each of its kZ heights).  kZ > 1 only for the
-// interpolating branch without sub-voxel sampling: a batch of views that are
-// all z-coherent takes bp_zcol_view, any other batch the per-voxel code.
+// consecutive voxels at each of its kZ heights).  The interpolating branch
+// without sub-voxel sampling runs bp_col_views, with the shared u part on a
+// batch of views that are all z-coherent; the host takes kZ > 1 only there.
+// A voxel's sum is the same for every kZ and launch shape.
 template <int kBy, int kZ, typename Tv, typename Tp, typename Tm>
 __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
     BpCfg c, const ap_proj_spec* __restrict__ specs, uint32_t num_projs,
@@ -609,19 +579,21 @@ __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
     const float x0 = c.c000[0] + (float)gi * c.w, x1 = c.c000[1] + (float)gj * c.w;
     bool col_in = cj < c.shp[1] && ci < c.shp[0];
     if (c.mask && x0 * x0 + x1 * x1 > c.mask_r2) col_in = false;
+    // the thread's voxels in the volume: bit m for voxel m (an integer, not
+    // flags: flags live across the view loop would occupy scalar registers)
     float z[kZ], acc[kZ];
-    bool act[kZ];
-    bool any = false;
+    uint32_t act = 0;
 #pragma unroll
     for (int m = 0; m < kZ; m++) {
         const uint32_t ck = ck0 + (uint32_t)(kBz * m);
         z[m] = c.c000[2] + (float)(ck + c.off[2]) * c.w;
-        act[m] = col_in && ck < c.shp[2];
+        if (col_in && ck < c.shp[2]) act |= 1u << m;
         acc[m] = 0.f;
-        any = any || act[m];
     }
+    asm volatile("" : "+v"(act));
+    const bool any = act != 0;
     const uint32_t nss = c.ssx * c.ssy * c.ssz;
-    const bool zcol = kZ > 1 && !c.exact && nss == 1 && (c.bilinear ? c.pair : c.off32);
+    const bool zcol = !c.exact && nss == 1 && (c.bilinear ? c.pair : c.off32);
     const int tid = threadIdx.y * kBz + threadIdx.x;
     for (uint32_t p0 = 0; p0 < num_projs; p0 += kViewBatch) {
         const uint32_t nb = min((uint32_t)kViewBatch, num_projs - p0);
@@ -634,38 +606,38 @@ __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
         // every view of the batch z-coherent (the barrier publishes the maps)
         const bool batch_z = __syncthreads_and(coherent) != 0;
         if (!any || nss != 1) continue;
-        if (zcol && batch_z) {
-            for (uint32_t q = 0; q < nb; q++)
-                bp_zcol_view<kZ>(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, x0, x1, z, act, acc);
+        if (zcol) {
+            const Tp* img = sino + (int64_t)p0 * c.s0;
+            uint32_t q = 0;
+            if constexpr (kZ == 1) {
+                if (c.grouped) {
+                    for (; q + 4 <= nb; q += 4) {
+                        if (batch_z)
+                            bp_col_views<4, 1, true>(c, maps + q, img + (int64_t)q * c.s0, c.s0, x0, x1, z, act, acc);
+                        else
+                            bp_col_views<4, 1, false>(c, maps + q, img + (int64_t)q * c.s0, c.s0, x0, x1, z, act, acc);
+                    }
+                }
+            }
+            for (; q < nb; q++) {
+                if (batch_z)
+                    bp_col_views<1, kZ, true>(c, maps + q, img + (int64_t)q * c.s0, c.s0, x0, x1, z, act, acc);
+                else
+                    bp_col_views<1, kZ, false>(c, maps + q, img + (int64_t)q * c.s0, c.s0, x0, x1, z, act, acc);
+            }
             continue;
         }
-        // per voxel: one copy of the view loop, the voxel's height and sum taken
-        // from (and put back into) its registers by selects
-#pragma unroll 1
-        for (int m = 0; m < kZ; m++) {
-            float zm = z[0], am = 0.f;
-            bool on = act[0];
-#pragma unroll
-            for (int i = 1; i < kZ; i++)
-                if (m == i) { zm = z[i]; on = act[i]; }
-            if (!on) continue;
-            const float xc[3] = {x0, x1, zm};
+        if constexpr (kZ == 1) {
+            // the exact branch, and the interpolating branch with unpaired taps or
+            // 64-bit detector offsets, voxel by voxel
+            const float xc[3] = {x0, x1, z[0]};
             if (c.exact) {
                 for (uint32_t q = 0; q < nb; q++)
-                    am += bp_one_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
-            } else if (c.grouped) {
-                uint32_t q = 0;
-                for (; q + 4 <= nb; q += 4)
-                    bp_fast_views4(c, maps + q, sino + (int64_t)(p0 + q) * c.s0, c.s0, xc, am);
-                for (; q < nb; q++)
-                    am += bp_fast_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
+                    acc[0] += bp_one_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
             } else {
                 for (uint32_t q = 0; q < nb; q++)
-                    am += bp_fast_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
+                    acc[0] += bp_fast_view(c, maps[q], sino + (int64_t)(p0 + q) * c.s0, xc);
             }
-#pragma unroll
-            for (int i = 0; i < kZ; i++)
-                if (m == i) acc[i] += am;
         }
     }
     if (nss != 1) {
@@ -673,7 +645,7 @@ __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
         // (CB_backproject.cu:377-383 loop order: x, y, z outermost to innermost)
 #pragma unroll
         for (int m = 0; m < kZ; m++) {
-            if (!act[m]) continue;
+            if (!((act >> m) & 1u)) continue;
             float ax = 0.f;
             for (uint32_t si = 0; si < c.ssx; si++) {
                 float ay = 0.f;
@@ -700,7 +672,7 @@ __global__ __launch_bounds__(kBz* kBy) void k_backproject_voxel(
     }
 #pragma unroll
     for (int m = 0; m < kZ; m++) {
-        if (!act[m] || acc[m] == 0.f) continue;
+        if (!((act >> m) & 1u) || acc[m] == 0.f) continue;
         const int64_t gk = (int64_t)(ck0 + (uint32_t)(kBz * m) + c.off[2]);
         float mval = 1.f;
         if (c.use_mult) mval = to_f32(mult[(int64_t)gi * ms0 + (int64_t)gj * ms1 + gk * ms2]);
@@ -924,9 +896,10 @@ int32_t launch_backproject_voxel(const ap_config_cb_backproject* cfg, const ap_p
     const int var = ap_tune_variant(AP_TUNE_BACKPROJECT);
     const uint32_t rows_y = (uint32_t)kBpRowsY[var];
     // four voxels per thread along z where a column is taller than two waves'
-    // worth and the branch can share a view's u part between them
+    // worth and the interpolating branch runs bp_col_views (which shares a
+    // z-coherent view's u part between them)
     const uint32_t nss = c.ssx * c.ssy * c.ssz;
-    const bool z4 = !c.exact && nss == 1 && c.shp[2] > 2 * (uint32_t)kBz;
+    const bool z4 = !c.exact && nss == 1 && (c.bilinear ? c.pair : c.off32) && c.shp[2] > 2 * (uint32_t)kBz;
     const uint32_t zper = z4 ? 4u * kBz : (uint32_t)kBz;
     dim3 block(kBz, rows_y, 1);
     dim3 grid((c.shp[2] + zper - 1) / zper, (c.shp[1] + rows_y - 1) / rows_y, c.shp[0]);
