@@ -143,6 +143,16 @@ __device__ void make_prim(const ap_forbild_shape& s, Prim& P) {
     set_bound(P, lr);
 }
 
+// 1 / x for a nonzero finite x: the hardware reciprocal refined by two Newton
+// steps (error about 1e-16 relative; an IEEE division is ten instructions)
+__device__ __forceinline__ double rcp64(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
 // stable roots of A t^2 + 2 B t + C = 0; false if no real interval
 __device__ __forceinline__ bool quad_roots(double A, double B, double C, double& t0, double& t1) {
     if (A == 0.0) return false;
@@ -150,7 +160,7 @@ __device__ __forceinline__ bool quad_roots(double A, double B, double C, double&
     if (!(disc > 0.0)) return false;
     const double sq = sqrt(disc);
     const double q = B > 0.0 ? -(B + sq) : -(B - sq);
-    const double r1 = q / A, r2 = q != 0.0 ? C / q : -r1;
+    const double r1 = q * rcp64(A), r2 = q != 0.0 ? C * rcp64(q) : -r1;
     t0 = fmin(r1, r2);
     t1 = fmax(r1, r2);
     return true;
@@ -169,7 +179,7 @@ __device__ bool prim_interval(const Prim& P, const d3& S, const d3& d, double& t
         if (sd == 0.0) {
             if (s0 > 0.0) return false;
         } else {
-            const double t = -s0 / sd;
+            const double t = -s0 * rcp64(sd);
             if (sd > 0.0) tb = fmin(tb, t); else ta = fmax(ta, t);
         }
     }
@@ -189,7 +199,7 @@ __device__ bool prim_interval(const Prim& P, const d3& S, const d3& d, double& t
                 if (fabs(ls[a]) > 1.0) return false;
                 continue;
             }
-            const double inv = 1.0 / ld[a];
+            const double inv = rcp64(ld[a]);
             const double x0 = (-1.0 - ls[a]) * inv, x1 = (1.0 - ls[a]) * inv;
             ta = fmax(ta, fmin(x0, x1));
             tb = fmin(tb, fmax(x0, x1));
@@ -207,7 +217,7 @@ __device__ bool prim_interval(const Prim& P, const d3& S, const d3& d, double& t
         if (ld[2] == 0.0) {
             if (fabs(ls[2]) > P.h) return false;
         } else {
-            const double inv = 1.0 / ld[2];
+            const double inv = rcp64(ld[2]);
             const double x0 = (-P.h - ls[2]) * inv, x1 = (P.h - ls[2]) * inv;
             ta = fmax(ta, fmin(x0, x1));
             tb = fmin(tb, fmax(x0, x1));
@@ -228,7 +238,7 @@ __device__ bool prim_interval(const Prim& P, const d3& S, const d3& d, double& t
         if (ld[2] == 0.0) {
             if (fabs(ls[2]) > P.h) return false;
         } else {
-            const double inv = 1.0 / ld[2];
+            const double inv = rcp64(ld[2]);
             const double x0 = (-P.h - ls[2]) * inv, x1 = (P.h - ls[2]) * inv;
             ta = fmax(ta, fmin(x0, x1));
             tb = fmin(tb, fmax(x0, x1));
@@ -245,9 +255,9 @@ __device__ bool prim_interval(const Prim& P, const d3& S, const d3& d, double& t
             if (B == 0.0) {
                 if (C > 0.0) return false;
             } else if (B > 0.0) {
-                tb = fmin(tb, -C / (2.0 * B));
+                tb = fmin(tb, -C * rcp64(2.0 * B));
             } else {
-                ta = fmax(ta, -C / (2.0 * B));
+                ta = fmax(ta, -C * rcp64(2.0 * B));
             }
         } else if (A > 0.0) {
             if (!quad_roots(A, B, C, t0, t1)) return false;
