@@ -122,3 +122,31 @@ def test_forward_propagates_infinite_voxel(device):
     np.testing.assert_array_equal(np.isinf(got), hit)
     assert (got[hit] > 0).all()
     assert rel_l2(got[~hit], want[~hit]) <= 1e-5
+
+
+def test_forward_infinite_voxel_on_faces(device):
+    # +inf voxels on a z face and on an x-y face of the volume.  Rays through
+    # them are non-finite as in the oracle.  The walk also reads face voxels with
+    # weight zero (padding segments, rows entering or leaving through a z face),
+    # so a few rays passing next to them turn NaN where the oracle stays finite
+    # (a documented deviation); every ray that stays finite matches the oracle.
+    rng = np.random.default_rng(44)
+    shape, w = (20, 18, 22), 0.6
+    pg = ph.circular_geometry(8, 30, 28, 1.0, R=30.0, L=60.0)
+    x = rng.uniform(0.2, 1.0, shape).astype(np.float32)
+    x[10, 9, 0] = np.inf     # first z plane
+    x[0, 9, 11] = np.inf     # first x plane
+    cfg = cbfp.ConfigProject()
+    ps = cbfp.ProjectionSet(pg)
+    cbfp.project(cfg, Volume(x, voxwid=w), ps, block=True)
+    got = ps.data.cpu().numpy()
+    want = orc.project_voxel(cfg.contents, pg.to_proj_spec_array(), chunk_for(shape, w), x,
+                             np.zeros((8, 30, 28)))
+    hit = np.isinf(want)
+    assert hit.any()
+    assert not np.isfinite(got[hit]).any()
+    fin = np.isfinite(got)
+    extra = (~fin & ~hit).sum()
+    print(f"face inf: {hit.sum()} rays through the voxels, {extra} more non-finite of {(~hit).sum()}")
+    assert extra <= 0.05 * (~hit).sum()
+    assert rel_l2(got[fin], want[fin]) <= 1e-5
