@@ -16,7 +16,8 @@ struct GView {
     float bx, by;        // step of the detector point per sub-column (x, y)
     float orient;        // sign of cross(b, a): cross(b, X - S) * orient > 0 in front
     int32_t t0;          // rows (z order) whose ray does not rise (dz <= 0); set by the gather pre-pass
-    float pad[3];
+    float szlo;          // sz's FP32 rounding residual: the source height is sz + szlo
+    float pad[2];
 };
 static_assert(sizeof(GView) == 64, "GView layout");
 
@@ -57,6 +58,7 @@ __device__ inline bool gather_view(const RayCfg& rc, const ap_proj_spec* __restr
     gv.ok = 1;
     gv.flip = flip;
     gv.sz = (float)Sz;
+    gv.szlo = (float)(Sz - (double)gv.sz);
     gv.dz0 = (float)(vr.Q00[2] + kv_first * vz - vr.S[2]);
     gv.dzs = (float)(fabs(vz) / ssv);
     gv.sx = (float)Sx;

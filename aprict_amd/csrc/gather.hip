@@ -259,6 +259,7 @@ struct GVu {
     float sz, sx, sy;
     int t0;
     float ax, ay, bx, by;
+    float szlo;   // the source height is sz + szlo (FP64 crossings use both)
 };
 
 template <typename Tv, typename Tm, int kW, int kC, int kJ, int kTX>
@@ -288,6 +289,7 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
     // ax, ay, bx, by -- read from LDS at the start of a step, so that no step waits
     // on a global load of its view record
     __shared__ __align__(16) float Vw[64 * 8];
+    __shared__ float VwLo[64];   // the views' szlo
     // per-wave (cell, column) pair records of the setup pass, one per lane:
     // l / dh, dh = h_b - h_a, alpha_a, alpha_b, 0, window offset | flags, H (FP64).
     // The records start 8 bytes past a 16-byte boundary: the compiler then reads
@@ -378,8 +380,11 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
         // k - sz per slot, exact in FP64 (k an integer below 2^24, sz a float)
         double kzd[kJ];
 #pragma unroll
-        for (int jj = 0; jj < kJ; jj++) kzd[jj] = (double)(z0 + lane + 64 * jj) - (double)g.sz;
-        const double kz1d = (double)z1 - (double)g.sz;
+        // k - sz per slot in FP64 with sz to FP64 precision (sz + szlo): a ray nearly
+        // parallel to the planes crosses plane k at (k - sz) l / dz, so an FP32 sz
+        // would move its crossing by up to 1e-5 l / dz voxels
+        for (int jj = 0; jj < kJ; jj++) kzd[jj] = ((double)(z0 + lane + 64 * jj) - (double)g.sz) - (double)g.szlo;
+        const double kz1d = ((double)z1 - (double)g.sz) - (double)g.szlo;
         const float kzspan = fmaxf(fabsf((float)z0 - g.sz), fabsf(zf1 - g.sz));
         const float Wm2 = (float)(W - 2);
         const float msz = -g.sz;
@@ -654,6 +659,7 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
         g.ay = u(v1.y);
         g.bx = u(v1.z);
         g.by = u(v1.w);
+        g.szlo = u(VwLo[idx]);
         return g;
     };
 
@@ -665,10 +671,12 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
             wn.qlo = 0; wn.qhi = -1; wn.tlo = 0; wn.W = 2; wn.kc = 0.f; wn.ncg = 1; wn.pad[0] = wn.pad[1] = 0;
             const uint32_t p = pb + (uint32_t)lane;
             float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+            float szlo = 0.f;
             if (p < gc.V && gviews[p].ok) {
                 const GView g = gviews[p];
                 v0 = make_float4(g.sz, g.sx, g.sy, __int_as_float(g.t0));
                 v1 = make_float4(g.ax, g.ay, g.bx, g.by);
+                szlo = g.szlo;
                 float qmin = AP_BIG, qmax = -AP_BIG, dmax = 0.f;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
@@ -717,6 +725,7 @@ __global__ __launch_bounds__(kW * 64) void k_gather_adjoint(
             Win[lane] = wn;
             reinterpret_cast<float4*>(Vw)[lane * 2] = v0;
             reinterpret_cast<float4*>(Vw)[lane * 2 + 1] = v1;
+            VwLo[lane] = szlo;
         }
         __syncthreads();
         const uint32_t pe = min(gc.V, pb + 64);
