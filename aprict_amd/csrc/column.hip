@@ -281,8 +281,12 @@ __device__ inline RowWalk row_setup(const Col2D& C, float h0, float h1, float kv
     return W;
 }
 
+// Held to 5 waves per SIMD (96 VGPRs): the walk's gathers and crossing chain
+// need the latency hiding (512^3 x 720 views: 4 waves 69.7 ms, 5 waves
+// 65.5 ms, 6 waves 69.3 ms, where the walk itself starts to spill).  The
+// set-up spills a few registers to scratch once per column.
 template <int kRowsF, typename Tv, typename Tp, typename OffT>
-__global__ __launch_bounds__(kRowsF) void k_project_columns(
+__global__ __launch_bounds__(kRowsF, 5) void k_project_columns(
     RayCfg rc, SinoCfgC sc, ColMap cm, float mba, uint32_t ssu, uint32_t ssv,
     const ap_proj_spec* __restrict__ specs, const Tv* __restrict__ vol, OffT vsx, OffT vsy, OffT vsz,
     Tp* __restrict__ sino, const Tp* __restrict__ b, uint32_t cap) {
