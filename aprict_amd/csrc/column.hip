@@ -281,11 +281,13 @@ __device__ inline RowWalk row_setup(const Col2D& C, float h0, float h1, float kv
     return W;
 }
 
-// Held to 5 waves per SIMD (96 VGPRs): the walk's gathers and crossing chain
-// need the latency hiding (512^3 x 720 views: 4 waves 69.7 ms, 5 waves
-// 65.5 ms, 6 waves 69.3 ms, where the walk itself starts to spill).  The
-// set-up spills a few registers to scratch once per column.
-template <int kRowsF, typename Tv, typename Tp, typename OffT>
+// The walk's gathers and crossing chain are latency that only more waves
+// hide.  kOne: one sample per pixel (ssu = ssv = 1), so nothing of the
+// column's set-up stays live across the walk for a next sub-sample: 50 VGPRs,
+// 8 waves per SIMD.  With sub-samples the kernel is held to 5 waves (96
+// VGPRs; the set-up spills 14 registers to scratch once per column).
+// 512^3 x 720 views, 128 rows: 4 waves 69.7 ms, 5 waves 65.4 ms, kOne 56.0 ms.
+template <int kRowsF, typename Tv, typename Tp, typename OffT, bool kOne>
 __global__ __launch_bounds__(kRowsF, 5) void k_project_columns(
     RayCfg rc, SinoCfgC sc, ColMap cm, float mba, uint32_t ssu, uint32_t ssv,
     const ap_proj_spec* __restrict__ specs, const Tv* __restrict__ vol, OffT vsx, OffT vsy, OffT vsz,
@@ -309,6 +311,7 @@ __global__ __launch_bounds__(kRowsF, 5) void k_project_columns(
     const ViewRays vr = make_view_rays(rc, specs, p);
     const int lo2[2] = {rc.lo[0], rc.lo[1]}, hi2[2] = {rc.hi[0], rc.hi[1]};
     float tot = 0.f;
+    if constexpr (kOne) ssu = ssv = 1;
     for (uint32_t su = 0; su < ssu; su++) {
         Col2D C;
         bool ok = col_setup(vr, rc.lo, rc.hi, (float)j + sub_off(su, ssu), C);
@@ -403,14 +406,20 @@ int32_t launch_project_columns(const RayCfg& rc, uint32_t V, uint32_t nu, uint32
                     };
                     // 64-bit element offsets, or 32-bit byte offsets
                     if (!bytes32) {
-                        if (var == 0) launch(k_project_columns<kColRows[0], Tv, Tp, int64_t>, int64_t{}, 1);
-                        if (var == 1) launch(k_project_columns<kColRows[1], Tv, Tp, int64_t>, int64_t{}, 1);
-                        if (var == 2) launch(k_project_columns<kColRows[2], Tv, Tp, int64_t>, int64_t{}, 1);
+                        if (var == 0) launch(k_project_columns<kColRows[0], Tv, Tp, int64_t, false>, int64_t{}, 1);
+                        if (var == 1) launch(k_project_columns<kColRows[1], Tv, Tp, int64_t, false>, int64_t{}, 1);
+                        if (var == 2) launch(k_project_columns<kColRows[2], Tv, Tp, int64_t, false>, int64_t{}, 1);
                     } else {
                         const uint32_t e = (uint32_t)sizeof(Tv);
-                        if (var == 0) launch(k_project_columns<kColRows[0], Tv, Tp, uint32_t>, uint32_t{}, e);
-                        if (var == 1) launch(k_project_columns<kColRows[1], Tv, Tp, uint32_t>, uint32_t{}, e);
-                        if (var == 2) launch(k_project_columns<kColRows[2], Tv, Tp, uint32_t>, uint32_t{}, e);
+                        if (ssu == 1 && ssv == 1) {
+                            if (var == 0) launch(k_project_columns<kColRows[0], Tv, Tp, uint32_t, true>, uint32_t{}, e);
+                            if (var == 1) launch(k_project_columns<kColRows[1], Tv, Tp, uint32_t, true>, uint32_t{}, e);
+                            if (var == 2) launch(k_project_columns<kColRows[2], Tv, Tp, uint32_t, true>, uint32_t{}, e);
+                        } else {
+                            if (var == 0) launch(k_project_columns<kColRows[0], Tv, Tp, uint32_t, false>, uint32_t{}, e);
+                            if (var == 1) launch(k_project_columns<kColRows[1], Tv, Tp, uint32_t, false>, uint32_t{}, e);
+                            if (var == 2) launch(k_project_columns<kColRows[2], Tv, Tp, uint32_t, false>, uint32_t{}, e);
+                        }
                     }
                     return ap_check_launch("project_columns");
                 }
