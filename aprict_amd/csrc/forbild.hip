@@ -401,11 +401,14 @@ __device__ double last_wins_exact(const ap_forbild_shape* __restrict__ shapes, u
     return val;
 }
 
-// 4 waves per SIMD, at most 128 VGPRs: the union state spills ~38 registers
-// to scratch, and the extra wave still hides more FP64 latency than the
-// spills cost (40 primitives x 720 x 512^2: 22.9 ms at 3 waves, 22.4 ms at 4)
-template <typename Tp>
-__global__ __launch_bounds__(kProjThreads, 4) void k_project_forbild(
+// The FP64 intersection chains are latency that more waves hide, even at the
+// cost of spilling part of the union state to scratch.  kOne: one ray per
+// pixel (ssu = ssv = 1), so the view's detector frame is dead once the ray is
+// formed: 7 waves per SIMD.  With sub-rays the frame stays live for the next
+// one: 4 waves.  40 primitives x 720 x 512^2: 22.9 ms at 3 waves (both
+// cases); kOne 18.2 ms at 4 waves (no spills), 16.3 at 6, 15.6 at 7, 22.8 at 8.
+template <typename Tp, bool kOne>
+__global__ __launch_bounds__(kProjThreads, kOne ? 7 : 4) void k_project_forbild(
     float mba, int additive, uint32_t ssu, uint32_t ssv, const ap_proj_spec* __restrict__ specs,
     const ap_forbild_shape* __restrict__ shapes, uint32_t K, uint32_t nu, uint32_t nv,
     int64_t s0, int64_t s1, int64_t s2, Tp* __restrict__ sino, uint32_t* __restrict__ oom) {
@@ -417,6 +420,7 @@ __global__ __launch_bounds__(kProjThreads, 4) void k_project_forbild(
     const ViewGeom g = load_view(specs, p);
     double tot = 0.0;
     bool overflow = false;
+    if constexpr (kOne) ssu = ssv = 1;
     for (uint32_t su = 0; su < ssu; su++) {
         double accu = 0.0;
         for (uint32_t sv = 0; sv < ssv; sv++) {
@@ -608,13 +612,16 @@ extern "C" int32_t ap_project_forbild(const ap_config_cb_project_forbild* cfg,
     AP_REQUIRE(npix < (1ull << 31), "detector too large");
     dim3 grid((uint32_t)((npix + kProjThreads - 1) / kProjThreads), num_projs, 1);
     hipStream_t s = (hipStream_t)stream;
+    const bool one = cfg->ssu == 1 && cfg->ssv == 1;
     if (sino_desc->dtype == AP_F32) {
-        hipLaunchKernelGGL(k_project_forbild<float>, grid, dim3(kProjThreads), 0, s,
+        auto kern = one ? k_project_forbild<float, true> : k_project_forbild<float, false>;
+        hipLaunchKernelGGL(kern, grid, dim3(kProjThreads), 0, s,
                            cfg->multiply_before_adding, (int)cfg->overlap_addition, cfg->ssu, cfg->ssv,
                            specs, shapes, num_shapes, nu, nv, sino_desc->strides[0],
                            sino_desc->strides[1], sino_desc->strides[2], (float*)sino, out_of_mem_count);
     } else if (sino_desc->dtype == AP_F64) {
-        hipLaunchKernelGGL(k_project_forbild<double>, grid, dim3(kProjThreads), 0, s,
+        auto kern = one ? k_project_forbild<double, true> : k_project_forbild<double, false>;
+        hipLaunchKernelGGL(kern, grid, dim3(kProjThreads), 0, s,
                            cfg->multiply_before_adding, (int)cfg->overlap_addition, cfg->ssu, cfg->ssv,
                            specs, shapes, num_shapes, nu, nv, sino_desc->strides[0],
                            sino_desc->strides[1], sino_desc->strides[2], (double*)sino, out_of_mem_count);

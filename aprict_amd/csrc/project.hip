@@ -61,7 +61,9 @@ __device__ __forceinline__ float sub_offset(uint32_t s, uint32_t ss) {
     return -0.5f + ((float)s + 0.5f) / (float)ss;
 }
 
-template <int kRowsU, int kBatch, typename Tv, typename Tp, typename IdxT>
+// kOne: one ray per pixel (ssu = ssv = 1), so the view's ray frame is dead
+// during the walk (fewer registers live across it).
+template <int kRowsU, int kBatch, typename Tv, typename Tp, typename IdxT, bool kOne>
 __global__ __launch_bounds__(kWaveV* kRowsU) void k_project_voxel(
     RayCfg rc, SinoCfg sc, TileMap tm, float mba, uint32_t ssu, uint32_t ssv,
     const ap_proj_spec* __restrict__ specs, const Tv* __restrict__ vol, IdxT vsx, IdxT vsy, IdxT vsz,
@@ -73,6 +75,7 @@ __global__ __launch_bounds__(kWaveV* kRowsU) void k_project_voxel(
     // 64th of one spec load each, not a workgroup launch or a serial load.
     // The test depends on threadIdx.x only, so every wave sees the same mask.
     const uint32_t lim = tm.per_xcd * kXcd;
+    if constexpr (kOne) ssu = ssv = 1;
     for (uint32_t L0 = blockIdx.x; L0 < lim; L0 += 64u * gridDim.x) {
         bool live;
         {
@@ -478,13 +481,17 @@ extern "C" int32_t ap_project_voxel(const ap_config_cb_project* cfg, const ap_pr
                                            (IdxT)vol_desc->strides[2], (Tp*)sino, (const Tp*)b, skip);
                     };
                     if (idx64) {
-                        if (var == 0) launch(k_project_voxel<kRayRowsU[0], kRayBatch[0], Tv, Tp, int64_t>, int64_t{});
-                        if (var == 1) launch(k_project_voxel<kRayRowsU[1], kRayBatch[1], Tv, Tp, int64_t>, int64_t{});
-                        if (var == 2) launch(k_project_voxel<kRayRowsU[2], kRayBatch[2], Tv, Tp, int64_t>, int64_t{});
+                        if (var == 0) launch(k_project_voxel<kRayRowsU[0], kRayBatch[0], Tv, Tp, int64_t, false>, int64_t{});
+                        if (var == 1) launch(k_project_voxel<kRayRowsU[1], kRayBatch[1], Tv, Tp, int64_t, false>, int64_t{});
+                        if (var == 2) launch(k_project_voxel<kRayRowsU[2], kRayBatch[2], Tv, Tp, int64_t, false>, int64_t{});
+                    } else if (cfg->ssu == 1 && cfg->ssv == 1) {
+                        if (var == 0) launch(k_project_voxel<kRayRowsU[0], kRayBatch[0], Tv, Tp, int32_t, true>, int32_t{});
+                        if (var == 1) launch(k_project_voxel<kRayRowsU[1], kRayBatch[1], Tv, Tp, int32_t, true>, int32_t{});
+                        if (var == 2) launch(k_project_voxel<kRayRowsU[2], kRayBatch[2], Tv, Tp, int32_t, true>, int32_t{});
                     } else {
-                        if (var == 0) launch(k_project_voxel<kRayRowsU[0], kRayBatch[0], Tv, Tp, int32_t>, int32_t{});
-                        if (var == 1) launch(k_project_voxel<kRayRowsU[1], kRayBatch[1], Tv, Tp, int32_t>, int32_t{});
-                        if (var == 2) launch(k_project_voxel<kRayRowsU[2], kRayBatch[2], Tv, Tp, int32_t>, int32_t{});
+                        if (var == 0) launch(k_project_voxel<kRayRowsU[0], kRayBatch[0], Tv, Tp, int32_t, false>, int32_t{});
+                        if (var == 1) launch(k_project_voxel<kRayRowsU[1], kRayBatch[1], Tv, Tp, int32_t, false>, int32_t{});
+                        if (var == 2) launch(k_project_voxel<kRayRowsU[2], kRayBatch[2], Tv, Tp, int32_t, false>, int32_t{});
                     }
                     return ap_check_launch("project_voxel");
                 }
